@@ -1,0 +1,289 @@
+"""ctypes front of helix_amd's native libraries.
+
+``libhelix_hip.so`` holds the gfx950 kernels behind the C-ABI of ``include/helix_hip.h``;
+``libhelix.so`` holds the C++17 host set-up (``include/helix_setup.h``) and the public C++ API.
+Nothing here computes physics: a missing library raises, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(PKG, "libhelix_hip.so")
+HOST_LIB = os.path.join(PKG, "libhelix.so")
+
+FP64, FP32, SPLIT_FP32 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "split_fp32": SPLIT_FP32}
+
+# every symbol include/helix_hip.h and include/helix_setup.h declare
+HIP_SYMBOLS = [
+    "helix_hip_plan_create", "helix_hip_set_initial", "helix_hip_set_state", "helix_hip_get_state", "helix_hip_rhs",
+    "helix_hip_step", "helix_hip_get_tier0", "helix_hip_time_steps", "helix_hip_plan_info", "helix_hip_plan_destroy",
+    "helix_hip_last_error",
+]
+SETUP_SYMBOLS = [
+    "helix_model_legacy", "helix_model_spin_glass", "helix_model_problem", "helix_model_rows",
+    "helix_model_energy_diagonal", "helix_model_energy_trace", "helix_model_free", "helix_setup_pade",
+    "helix_setup_spin_glass_couplings", "helix_setup_hierarchy_size", "helix_setup_last_error",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class SpinOperator(C.Structure):
+    _fields_ = [("diag", _dp), ("nflips", C.c_int32), ("flip_masks", _ip), ("flip_coefs", _dp)]
+
+
+class Problem(C.Structure):
+    """Mirror of helix_hip_problem (read-only use from Python)."""
+    _fields_ = [
+        ("abi_version", C.c_int32), ("n", C.c_int32), ("n_ado", C.c_int32), ("n_modes", C.c_int32),
+        ("n_baths", C.c_int32), ("up", _ip), ("down", _ip), ("mode_bath", _ip), ("tier", _ip),
+        ("down_comm", _dp), ("down_anti", _dp), ("self_coef", _dp), ("xi", _dp), ("xi_counter", _dp),
+        ("hamiltonian", SpinOperator), ("hamiltonian_dense", _dp), ("couplings", C.POINTER(SpinOperator)),
+        ("precision", C.c_int32), ("order", C.c_int32), ("dt", C.c_double),
+    ]
+
+
+_hip = None
+_host = None
+
+
+class HelixError(RuntimeError):
+    pass
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise HelixError(f"{os.path.basename(path)} is not built; run `python -m helix_amd.build`")
+    return C.CDLL(path)
+
+
+def hip():
+    """libhelix_hip.so with argtypes bound."""
+    global _hip
+    if _hip is None:
+        L = _load(HIP_LIB)
+        vp = C.c_void_p
+        L.helix_hip_plan_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.helix_hip_set_initial.argtypes = [vp, _dp]
+        L.helix_hip_set_state.argtypes = [vp, _dp]
+        L.helix_hip_get_state.argtypes = [vp, _dp]
+        L.helix_hip_rhs.argtypes = [vp, _dp]
+        L.helix_hip_step.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
+        L.helix_hip_get_tier0.argtypes = [vp, _dp]
+        L.helix_hip_time_steps.argtypes = [vp, C.c_uint64, _dp, _dp]
+        L.helix_hip_plan_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
+        L.helix_hip_plan_destroy.argtypes = [vp]
+        L.helix_hip_plan_destroy.restype = None
+        L.helix_hip_last_error.restype = C.c_char_p
+        _hip = L
+    return _hip
+
+
+def host():
+    """libhelix.so (set-up + public API) with argtypes bound."""
+    global _host
+    if _host is None:
+        hip()  # resolve the kernel library first
+        L = _load(HOST_LIB)
+        vp = C.c_void_p
+        L.helix_model_legacy.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_double, C.c_int, C.POINTER(vp)]
+        L.helix_model_spin_glass.argtypes = [C.c_int, C.c_uint64, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int,
+                                             C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+        L.helix_model_problem.argtypes = [vp]
+        L.helix_model_problem.restype = C.POINTER(Problem)
+        L.helix_model_rows.argtypes = [vp, _ip]
+        L.helix_model_energy_diagonal.argtypes = [vp, _dp]
+        L.helix_model_energy_diagonal.restype = C.c_double
+        L.helix_model_energy_trace.argtypes = [vp, _dp]
+        L.helix_model_energy_trace.restype = C.c_double
+        L.helix_model_free.argtypes = [vp]
+        L.helix_model_free.restype = None
+        L.helix_setup_pade.argtypes = [C.c_int, _dp, _dp]
+        L.helix_setup_spin_glass_couplings.argtypes = [C.c_int, C.c_uint64, _dp]
+        L.helix_setup_hierarchy_size.argtypes = [C.c_int, C.c_int]
+        L.helix_setup_hierarchy_size.restype = C.c_long
+        L.helix_setup_last_error.restype = C.c_char_p
+        _host = L
+    return _host
+
+
+def _pd(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc: int, what: str, lib=None):
+    if rc != 0:
+        msg = (lib or hip()).helix_hip_last_error() if lib is None else lib.helix_setup_last_error()
+        raise HelixError(f"{what} failed (rc={rc}): {msg.decode() if msg else ''}")
+
+
+class Model:
+    """A tabulated HEOM problem built by the C++ host set-up."""
+
+    def __init__(self, handle, N: int):
+        self._h = handle
+        self.N = N
+        p = host().helix_model_problem(self._h).contents
+        self.H = p.n_ado
+        self.M = p.n_modes
+        self.B = p.n_baths
+        self.order = p.order
+        self.dt = p.dt
+        self.precision = p.precision
+
+    @classmethod
+    def legacy(cls, N=1024, beta=3.0, gamma=1.0, zeta=0.01, K=2, tiers=3, order=4, dt=0.1, precision="fp64"):
+        h = C.c_void_p()
+        rc = host().helix_model_legacy(N, beta, gamma, zeta, K, tiers, order, dt, PRECISIONS[precision], C.byref(h))
+        _check(rc, "helix_model_legacy", host())
+        return cls(h, N)
+
+    @classmethod
+    def spin_glass(cls, spins, seed, lam=0.1, gamma=1.0, T=1.0, K=1, L=4, h=1.0, order=4, dt=0.01,
+                   precision="fp64", bath_params=None):
+        if bath_params is None:
+            bath_params = [(lam, gamma, T)] * spins
+        lams = np.array([b[0] for b in bath_params], np.float64)
+        gams = np.array([b[1] for b in bath_params], np.float64)
+        temps = np.array([b[2] for b in bath_params], np.float64)
+        hdl = C.c_void_p()
+        rc = host().helix_model_spin_glass(spins, seed, h, _pd(lams), _pd(gams), _pd(temps), K, L, order, dt,
+                                           PRECISIONS[precision], C.byref(hdl))
+        _check(rc, "helix_model_spin_glass", host())
+        return cls(hdl, 1 << spins)
+
+    def problem_ptr(self):
+        return host().helix_model_problem(self._h)
+
+    def rows(self) -> np.ndarray:
+        r = np.zeros((self.H, self.M), np.int32)
+        host().helix_model_rows(self._h, r.ctypes.data_as(_ip))
+        return r
+
+    def tables(self) -> dict:
+        p = self.problem_ptr().contents
+        H, M, B, N = self.H, self.M, self.B, self.N
+
+        def arr(ptr, n, dt=np.float64):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt).copy()
+        out = dict(up=arr(p.up, H * M, np.int32).reshape(H, M), down=arr(p.down, H * M, np.int32).reshape(H, M),
+                   mode_bath=arr(p.mode_bath, M, np.int32), tier=arr(p.tier, H, np.int32),
+                   down_comm=arr(p.down_comm, 2 * H * M).view(np.complex128).reshape(H, M),
+                   down_anti=arr(p.down_anti, 2 * H * M).view(np.complex128).reshape(H, M),
+                   self_coef=arr(p.self_coef, 2 * H).view(np.complex128),
+                   xi=arr(p.xi, 2 * B).view(np.complex128), xi_counter=arr(p.xi_counter, 2 * B).view(np.complex128),
+                   h_diag=arr(p.hamiltonian.diag, N))
+        nf = p.hamiltonian.nflips
+        out["h_masks"] = arr(p.hamiltonian.flip_masks, nf, np.int32) if nf else np.zeros(0, np.int32)
+        out["h_coefs"] = arr(p.hamiltonian.flip_coefs, nf) if nf else np.zeros(0)
+        return out
+
+    def energy_diagonal(self, rho0: np.ndarray) -> float:
+        r = np.ascontiguousarray(rho0, np.complex128)
+        return host().helix_model_energy_diagonal(self._h, _pd(r.view(np.float64)))
+
+    def energy(self, rho0: np.ndarray) -> float:
+        r = np.ascontiguousarray(rho0, np.complex128)
+        return host().helix_model_energy_trace(self._h, _pd(r.view(np.float64)))
+
+    def initial_rho(self) -> np.ndarray:
+        r = np.zeros((self.N, self.N), np.complex128)
+        r[self.N - 1, self.N - 1] = 1.0
+        return r
+
+    def close(self):
+        if self._h:
+            host().helix_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A device-resident HEOM propagator (helix_hip_plan)."""
+
+    def __init__(self, model: Model, device: int = 0):
+        self.model = model
+        self.N, self.H = model.N, model.H
+        self._p = C.c_void_p()
+        _check(hip().helix_hip_plan_create(C.cast(model.problem_ptr(), C.c_void_p), device, C.byref(self._p)),
+               "helix_hip_plan_create")
+
+    def set_initial(self, rho0: np.ndarray | None = None):
+        r = self.model.initial_rho() if rho0 is None else np.ascontiguousarray(rho0, np.complex128)
+        _check(hip().helix_hip_set_initial(self._p, _pd(r.view(np.float64))), "helix_hip_set_initial")
+
+    def set_state(self, state: np.ndarray):
+        s = np.ascontiguousarray(state, np.complex128)
+        assert s.shape == (self.H, self.N, self.N)
+        _check(hip().helix_hip_set_state(self._p, _pd(s.view(np.float64))), "helix_hip_set_state")
+
+    def get_state(self) -> np.ndarray:
+        s = np.zeros((self.H, self.N, self.N), np.complex128)
+        _check(hip().helix_hip_get_state(self._p, _pd(s.view(np.float64))), "helix_hip_get_state")
+        return s
+
+    def rhs(self) -> np.ndarray:
+        s = np.zeros((self.H, self.N, self.N), np.complex128)
+        _check(hip().helix_hip_rhs(self._p, _pd(s.view(np.float64))), "helix_hip_rhs")
+        return s
+
+    def step(self, nsteps: int, record_every: int = 0, record_rho: bool = False, record_diag: bool = False):
+        nrec = nsteps // record_every if record_every else 0
+        rho = np.zeros((nrec, self.N, self.N), np.complex128) if (nrec and record_rho) else None
+        diag = np.zeros((nrec, self.N)) if (nrec and record_diag) else None
+        _check(hip().helix_hip_step(self._p, nsteps, record_every,
+                                    _pd(rho.view(np.float64)) if rho is not None else None,
+                                    _pd(diag) if diag is not None else None), "helix_hip_step")
+        return rho, diag
+
+    def tier0(self) -> np.ndarray:
+        r = np.zeros((self.N, self.N), np.complex128)
+        _check(hip().helix_hip_get_tier0(self._p, _pd(r.view(np.float64))), "helix_hip_get_tier0")
+        return r
+
+    def time_steps(self, nsteps: int):
+        st = np.zeros(self.model.order)
+        tot = C.c_double()
+        _check(hip().helix_hip_time_steps(self._p, nsteps, _pd(st), C.byref(tot)), "helix_hip_time_steps")
+        return st, tot.value
+
+    def info(self) -> dict:
+        b, l, f, d = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+        _check(hip().helix_hip_plan_info(self._p, C.byref(b), C.byref(l), C.byref(f), C.byref(d)), "plan_info")
+        return dict(device_bytes=b.value, launches_per_step=l.value, flip=bool(f.value), dense=bool(d.value))
+
+    def close(self):
+        if self._p:
+            hip().helix_hip_plan_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pade(K: int):
+    p, r = np.zeros(K), np.zeros(K)
+    _check(host().helix_setup_pade(K, _pd(p), _pd(r)), "helix_setup_pade", host())
+    return p, r
+
+
+def spin_glass_couplings(spins: int, seed: int) -> np.ndarray:
+    J = np.zeros((spins, spins))
+    _check(host().helix_setup_spin_glass_couplings(spins, seed, _pd(J)), "couplings", host())
+    return J

@@ -1,0 +1,52 @@
+// heom_kernels.hpp -- gfx950 kernels of the HEOM Liouville action and its Taylor-m step.
+//
+// Layout in HBM: a state vector is [H][N][N] complex, row-major blocks, ADOs in the hierarchy's
+// lexicographic order (src/initialize_detail.h:174-214).  No sentinel block is stored: absent
+// neighbours are skipped by index (the reference reads a zero block, src/initialize.cu:289,306).
+// A Taylor stage j of order m is one fused launch (diagonal couplings) or two launches (couplings
+// with bit flips): it evaluates L(cur) and applies the recurrence in its epilogue,
+//   j <  m : t_j = (dt/j) L(t_{j-1})                          (t_0 = rho)
+//   j == m : rho = ((rho + t_1) + ... + t_{m-1}) + (dt/m) L(t_{m-1})
+// which keeps the reference's summation order F = rho; F += t_j (src/liouville.cu:525-590) without
+// storing F, and replaces cublasScal/cublasAxpy and both D2D copies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace helix_amd {
+
+constexpr int kMaxFlips = 16;
+constexpr int kMaxOrder = 8;
+constexpr int kBlock = 256;
+
+// Everything a stage launch needs; passed by value as the kernel argument block.
+struct StageArgs {
+	const void* cur;          // t_{j-1} (rho for j = 1)
+	void* out;                // t_j, or the new rho for the last stage
+	const void* rho;          // last stage: the old rho (may alias out)
+	const void* tprev[kMaxOrder]; // last stage: t_1 .. t_{m-1}
+	int ntprev;
+	double* rho0_f64;         // FP32 forms: tier-0 master copy in FP64 (last stage), else null
+	const void* buf;          // flip couplings: [B][H][N][N] = [V_b, cur_n]
+	const int32_t* up;        // [H][M]
+	const int32_t* down;      // [H][M]
+	const int32_t* mode_bath; // [M]
+	const double2* dcomm;     // [H][M]
+	const double2* danti;     // [H][M]
+	const double2* selfc;     // [H]
+	const double2* xi;        // [B]
+	const double2* xic;       // [B]
+	const double* vdiag;      // [B][N]
+	const int32_t* vflip_n;   // [B] flips of V_b (flip couplings only)
+	const int32_t* vflip_mask;// [B][kMaxFlips]
+	const double* vflip_coef; // [B][kMaxFlips]
+	const double* hdiag;      // [N]
+	int hflips;
+	int hmask[kMaxFlips];
+	double hcoef[kMaxFlips];
+	int n_ado, M, B, logN;
+	double scale;             // dt / j
+};
+
+}  // namespace helix_amd

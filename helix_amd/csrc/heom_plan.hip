@@ -1,0 +1,720 @@
+// heom_plan.hip -- gfx950 HEOM propagator behind the C-ABI of include/helix_hip.h.
+//
+// Replaces the reference's per-ADO stream/cuBLAS/cuSPARSE orchestration (src/liouville.cu:685-869),
+// its transpose kernel (src/matrix_util.cu:27-81) and the develop() scal/axpy/D2D sequence
+// (src/liouville.cu:502-591) by one launch per Taylor stage (two with bit-flip couplings).
+#include "heom_kernels.hpp"
+
+#include "helix_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace helix_amd {
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+	g_last_error = msg;
+	return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+	g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+	return static_cast<int>(e);
+}
+}  // namespace
+
+#define HX_HIP(call)                                         \
+	do {                                                     \
+		hipError_t hx_e_ = (call);                           \
+		if(hx_e_ != hipSuccess) return hip_fail(hx_e_, #call); \
+	} while(0)
+
+template <typename T>
+struct C2t;
+template <>
+struct C2t<double> {
+	using type = double2;
+};
+template <>
+struct C2t<float> {
+	using type = float2;
+};
+
+// acc += w * x for a complex weight w given in FP64.  SPLIT (FP32 only) applies w as the two-term
+// sum w_hi + w_lo of FP32 values so operator values keep ~FP64 accuracy while products and
+// accumulation stay in FP32.
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void cmadd(T& ar, T& ai, double wr, double wi, T xr, T xi)
+{
+	if constexpr(SPLIT) {
+		const float rh = static_cast<float>(wr), ih = static_cast<float>(wi);
+		const float rl = static_cast<float>(wr - static_cast<double>(rh));
+		const float il = static_cast<float>(wi - static_cast<double>(ih));
+		ar += rl * xr - il * xi;
+		ai += rl * xi + il * xr;
+		ar += rh * xr - ih * xi;
+		ai += rh * xi + ih * xr;
+	} else {
+		const T r = static_cast<T>(wr), m = static_cast<T>(wi);
+		ar += r * xr - m * xi;
+		ai += r * xi + m * xr;
+	}
+}
+
+// [V_b, X]_e and {V_b, X}_e for a bit-flip coupling V_b = diag(v) + sum_f c_f X_f:
+//   [V,X]_ij = (v_i - v_j) X_ij + sum_f c_f (X_{i^m,j} - X_{i,j^m})
+//   {V,X}_ij = (v_i + v_j) X_ij + sum_f c_f (X_{i^m,j} + X_{i,j^m})
+// (the row-major result of CommutateSparse / addAntiCommutateSparse, src/liouville.cu:275-366).
+template <typename T, bool SPLIT, bool ANTI>
+__device__ __forceinline__ void flip_comm_anti(const typename C2t<T>::type* __restrict__ X, const StageArgs& a,
+	int b, uint32_t i, uint32_t j, uint32_t e, T& cr, T& ci, T& sr, T& si)
+{
+	const uint32_t N = 1u << a.logN;
+	const double vi = a.vdiag[b * N + i], vj = a.vdiag[b * N + j];
+	const auto x = X[e];
+	cr = ci = sr = si = T(0);
+	cmadd<T, SPLIT>(cr, ci, vi - vj, 0.0, x.x, x.y);
+	if constexpr(ANTI) cmadd<T, SPLIT>(sr, si, vi + vj, 0.0, x.x, x.y);
+	const int nf = a.vflip_n[b];
+	for(int f = 0; f < nf; ++f) {
+		const uint32_t m = static_cast<uint32_t>(a.vflip_mask[b * kMaxFlips + f]);
+		const double cf = a.vflip_coef[b * kMaxFlips + f];
+		const auto x1 = X[e ^ (m << a.logN)];
+		const auto x2 = X[e ^ m];
+		cmadd<T, SPLIT>(cr, ci, cf, 0.0, x1.x - x2.x, x1.y - x2.y);
+		if constexpr(ANTI) cmadd<T, SPLIT>(sr, si, cf, 0.0, x1.x + x2.x, x1.y + x2.y);
+	}
+}
+
+// Pass A of a stage with bit-flip couplings: buf[b][n] = [V_b, cur_n]  (src/liouville.cu:792-801).
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(kBlock) void k_flip_commutator(const StageArgs a)
+{
+	using V2 = typename C2t<T>::type;
+	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
+	V2* __restrict__ buf = static_cast<V2*>(const_cast<void*>(a.buf));
+	const uint32_t logE = 2u * a.logN;
+	const uint32_t E = 1u << logE;
+	const uint32_t N = 1u << a.logN;
+	const uint32_t total = static_cast<uint32_t>(a.B * a.n_ado) << logE;
+	for(uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+		const uint32_t bn = g >> logE;
+		const uint32_t e = g & (E - 1);
+		const int b = static_cast<int>(bn / a.n_ado);
+		const uint32_t n = bn - static_cast<uint32_t>(b) * a.n_ado;
+		T cr, ci, sr, si;
+		flip_comm_anti<T, SPLIT, false>(cur + (static_cast<size_t>(n) << logE), a, b, e >> a.logN, e & (N - 1), e, cr,
+			ci, sr, si);
+		buf[g] = V2{cr, ci};
+	}
+}
+
+// One Taylor stage: L(cur) for every ADO element with the recurrence epilogue.
+// FLIP=false: every V_b diagonal, so [V_b, X] and {V_b, X} are elementwise and the whole RHS is one pass.
+// FLIP=true : V_b with bit flips; [V_b, cur] comes from pass A (buf), {V_b, .} terms are gathered here.
+template <typename T, bool SPLIT, bool LAST, bool FLIP>
+__global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
+{
+	using V2 = typename C2t<T>::type;
+	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
+	const V2* __restrict__ buf = static_cast<const V2*>(a.buf);
+	const int logN = a.logN;
+	const uint32_t N = 1u << logN;
+	const uint32_t logE = 2u * logN;
+	const uint32_t E = 1u << logE;
+	const uint32_t total = static_cast<uint32_t>(a.n_ado) << logE;
+	const int M = a.M;
+	for(uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < total; g += gridDim.x * kBlock) {
+		const uint32_t n = g >> logE;
+		const uint32_t e = g & (E - 1);
+		const uint32_t i = e >> logN, j = e & (N - 1);
+		const V2* __restrict__ cn = cur + (static_cast<size_t>(n) << logE);
+		const V2 c = cn[e];
+		T ar = T(0), ai = T(0);
+
+		// -i[H, rho_n] (diagonal part as in src/liouville.cu:368-386, plus bit-flip terms)
+		{
+			T hr = T(0), hi = T(0);
+			cmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], 0.0, c.x, c.y);
+			for(int f = 0; f < a.hflips; ++f) {
+				const uint32_t m = static_cast<uint32_t>(a.hmask[f]);
+				const V2 x1 = cn[e ^ (m << logN)];
+				const V2 x2 = cn[e ^ m];
+				cmadd<T, SPLIT>(hr, hi, a.hcoef[f], 0.0, x1.x - x2.x, x1.y - x2.y);
+			}
+			ar += hi;
+			ai -= hr;
+		}
+
+		const int32_t* __restrict__ upn = a.up + static_cast<size_t>(n) * M;
+		const int32_t* __restrict__ dnn = a.down + static_cast<size_t>(n) * M;
+
+		// phi: (-i) [V_{b(k)}, rho_{n+e_k}]
+		for(int k = 0; k < M; ++k) {
+			const int u = upn[k];
+			if(u >= a.n_ado) continue;
+			const int b = a.mode_bath[k];
+			if constexpr(!FLIP) {
+				const double d = a.vdiag[b * N + i] - a.vdiag[b * N + j];
+				const V2 x = cur[(static_cast<size_t>(u) << logE) + e];
+				cmadd<T, SPLIT>(ar, ai, 0.0, -d, x.x, x.y);
+			} else {
+				const V2 x = buf[(static_cast<size_t>(b * a.n_ado + u) << logE) + e];
+				ar += x.y;
+				ai -= x.x;
+			}
+		}
+
+		// psi / theta / theta2: down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}}
+		for(int k = 0; k < M; ++k) {
+			const int dn = dnn[k];
+			if(dn >= a.n_ado) continue;
+			const int b = a.mode_bath[k];
+			const double2 wc = a.dcomm[static_cast<size_t>(n) * M + k];
+			const double2 wa = a.danti[static_cast<size_t>(n) * M + k];
+			if constexpr(!FLIP) {
+				const double vi = a.vdiag[b * N + i], vj = a.vdiag[b * N + j];
+				const double d = vi - vj, s = vi + vj;
+				const V2 x = cur[(static_cast<size_t>(dn) << logE) + e];
+				cmadd<T, SPLIT>(ar, ai, wc.x * d + wa.x * s, wc.y * d + wa.y * s, x.x, x.y);
+			} else {
+				const V2 x = buf[(static_cast<size_t>(b * a.n_ado + dn) << logE) + e];
+				cmadd<T, SPLIT>(ar, ai, wc.x, wc.y, x.x, x.y);
+				if(wa.x != 0.0 || wa.y != 0.0) {
+					T cr, ci, sr, si;
+					flip_comm_anti<T, SPLIT, true>(cur + (static_cast<size_t>(dn) << logE), a, b, i, j, e, cr, ci, sr, si);
+					cmadd<T, SPLIT>(ar, ai, wa.x, wa.y, sr, si);
+				}
+			}
+		}
+
+		// Sigma, Xi and counter terms
+		{
+			const double2 sc = a.selfc[n];
+			if constexpr(!FLIP) {
+				double wr = sc.x, wi = sc.y;
+				for(int b = 0; b < a.B; ++b) {
+					const double vi = a.vdiag[b * N + i], vj = a.vdiag[b * N + j];
+					const double d = vi - vj, s = vi + vj;
+					wr -= a.xi[b].x * d * d + a.xic[b].x * s * d;
+					wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
+				}
+				cmadd<T, SPLIT>(ar, ai, wr, wi, c.x, c.y);
+			} else {
+				cmadd<T, SPLIT>(ar, ai, sc.x, sc.y, c.x, c.y);
+				for(int b = 0; b < a.B; ++b) {
+					T cr, ci, sr, si;
+					flip_comm_anti<T, SPLIT, true>(buf + (static_cast<size_t>(b * a.n_ado + n) << logE), a, b, i, j, e, cr,
+						ci, sr, si);
+					cmadd<T, SPLIT>(ar, ai, -a.xi[b].x, -a.xi[b].y, cr, ci);
+					cmadd<T, SPLIT>(ar, ai, -a.xic[b].x, -a.xic[b].y, sr, si);
+				}
+			}
+		}
+
+		// Taylor epilogue
+		const T s = static_cast<T>(a.scale);
+		const T tr = ar * s, ti = ai * s;
+		V2* __restrict__ out = static_cast<V2*>(a.out);
+		if constexpr(!LAST) {
+			out[g] = V2{tr, ti};
+		} else {
+			if(a.rho0_f64 != nullptr && n == 0) {
+				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
+				double2 r = r0[e];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					r.x += static_cast<double>(t.x);
+					r.y += static_cast<double>(t.y);
+				}
+				r.x += static_cast<double>(tr);
+				r.y += static_cast<double>(ti);
+				r0[e] = r;
+				out[g] = V2{static_cast<T>(r.x), static_cast<T>(r.y)};
+			} else {
+				V2 r = static_cast<const V2*>(a.rho)[g];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					r.x += t.x;
+					r.y += t.y;
+				}
+				r.x += tr;
+				r.y += ti;
+				out[g] = r;
+			}
+		}
+	}
+}
+
+// tier-0 block -> trace buffers (FP64): full block and/or real diagonal
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, double* rho_out,
+	double* diag_out)
+{
+	using V2 = typename C2t<T>::type;
+	const uint32_t N = 1u << logN;
+	const uint32_t E = N * N;
+	for(uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
+		double re, im;
+		if(rho0_f64) {
+			re = rho0_f64[2 * e];
+			im = rho0_f64[2 * e + 1];
+		} else {
+			const V2 v = static_cast<const V2*>(tier0)[e];
+			re = static_cast<double>(v.x);
+			im = static_cast<double>(v.y);
+		}
+		if(rho_out) {
+			rho_out[2 * e] = re;
+			rho_out[2 * e + 1] = im;
+		}
+		if(diag_out && (e >> logN) == (e & (N - 1))) diag_out[e >> logN] = re;
+	}
+}
+
+}  // namespace helix_amd
+
+using namespace helix_amd;
+
+struct helix_hip_plan {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int N = 0, logN = 0, H = 0, M = 0, B = 0, order = 4, precision = 0;
+	double dt = 0.0;
+	bool flip = false;
+	size_t esize = 16;        // bytes per complex element
+	size_t elems = 0;         // H * N * N
+	void* rho = nullptr;
+	void* t[kMaxOrder + 1] = {};
+	double* rho0_f64 = nullptr;
+	void* buf = nullptr;
+	std::vector<void*> allocations;
+	size_t device_bytes = 0;
+	StageArgs base{};
+	std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+int plan_alloc(helix_hip_plan* p, void** ptr, size_t bytes)
+{
+	*ptr = nullptr;
+	if(bytes == 0) return 0;
+	hipError_t e = hipMalloc(ptr, bytes);
+	if(e != hipSuccess) return hip_fail(e, "hipMalloc");
+	p->allocations.push_back(*ptr);
+	p->device_bytes += bytes;
+	return 0;
+}
+
+template <typename X>
+int upload(helix_hip_plan* p, const X* host, size_t count, const X** dev)
+{
+	void* d = nullptr;
+	int rc = plan_alloc(p, &d, count * sizeof(X));
+	if(rc) return rc;
+	if(count) HX_HIP(hipMemcpy(d, host, count * sizeof(X), hipMemcpyHostToDevice));
+	*dev = static_cast<const X*>(d);
+	return 0;
+}
+
+int ilog2_exact(int v)
+{
+	int l = 0;
+	while((1 << l) < v) ++l;
+	return ((1 << l) == v) ? l : -1;
+}
+
+int validate(const helix_hip_problem* pr)
+{
+	if(!pr) return fail(HELIX_HIP_EINVAL, "problem is null");
+	if(pr->abi_version != HELIX_HIP_ABI_VERSION) return fail(HELIX_HIP_EINVAL, "abi_version mismatch");
+	const int logN = ilog2_exact(pr->n);
+	if(logN < 3 || logN > 12) return fail(HELIX_HIP_EINVAL, "n must be a power of two in [8, 4096]");
+	if(pr->n_ado <= 0 || pr->n_modes <= 0 || pr->n_baths <= 0 || pr->n_modes > 64 || pr->n_baths > 64)
+		return fail(HELIX_HIP_EINVAL, "hierarchy sizes out of range");
+	const unsigned long long elems = static_cast<unsigned long long>(pr->n_ado) << (2 * logN);
+	if(elems * static_cast<unsigned long long>(pr->n_baths) >= (1ull << 31))
+		return fail(HELIX_HIP_EINVAL, "state too large for 32-bit element indexing");
+	if(!pr->up || !pr->down || !pr->mode_bath || !pr->tier || !pr->down_comm || !pr->down_anti || !pr->self_coef ||
+		!pr->xi || !pr->xi_counter || !pr->couplings || !pr->hamiltonian.diag)
+		return fail(HELIX_HIP_EINVAL, "problem table pointer is null");
+	if(pr->order < 1 || pr->order > kMaxOrder) return fail(HELIX_HIP_EINVAL, "order must be in [1, 8]");
+	if(!(pr->dt > 0.0)) return fail(HELIX_HIP_EINVAL, "dt must be positive");
+	if(pr->precision < HELIX_HIP_FP64 || pr->precision > HELIX_HIP_SPLIT_FP32)
+		return fail(HELIX_HIP_EINVAL, "unknown precision");
+	if(pr->hamiltonian_dense) return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path not built in this library");
+	auto check_op = [&](const helix_hip_spin_operator& op, const char* name) -> int {
+		if(op.nflips < 0 || op.nflips > kMaxFlips) return fail(HELIX_HIP_EINVAL, std::string(name) + ": nflips > 16");
+		for(int f = 0; f < op.nflips; ++f)
+			if(op.flip_masks[f] <= 0 || op.flip_masks[f] >= pr->n)
+				return fail(HELIX_HIP_EINVAL, std::string(name) + ": flip mask outside (0, n)");
+		return 0;
+	};
+	if(int rc = check_op(pr->hamiltonian, "hamiltonian")) return rc;
+	for(int b = 0; b < pr->n_baths; ++b) {
+		if(!pr->couplings[b].diag) return fail(HELIX_HIP_EINVAL, "coupling diag is null");
+		if(int rc = check_op(pr->couplings[b], "coupling")) return rc;
+	}
+	for(long q = 0; q < static_cast<long>(pr->n_ado) * pr->n_modes; ++q) {
+		if(pr->up[q] < 0 || pr->up[q] > pr->n_ado || pr->down[q] < 0 || pr->down[q] > pr->n_ado)
+			return fail(HELIX_HIP_EINVAL, "up/down index outside [0, n_ado]");
+	}
+	for(int k = 0; k < pr->n_modes; ++k)
+		if(pr->mode_bath[k] < 0 || pr->mode_bath[k] >= pr->n_baths) return fail(HELIX_HIP_EINVAL, "mode_bath out of range");
+	if(pr->tier[0] != 0) return fail(HELIX_HIP_EINVAL, "ADO 0 must be the tier-0 ADO");
+	return 0;
+}
+
+unsigned grid_for(size_t total)
+{
+	size_t g = (total + kBlock - 1) / kBlock;
+	if(g > (1u << 30)) g = 1u << 30;
+	return static_cast<unsigned>(g ? g : 1);
+}
+
+template <typename T, bool SPLIT>
+void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	const unsigned grid = grid_for(p->elems);
+	if(p->flip) {
+		hipLaunchKernelGGL((k_flip_commutator<T, SPLIT>), dim3(grid_for(p->elems * p->B)), dim3(kBlock), 0, p->stream, a);
+		if(last)
+			hipLaunchKernelGGL((k_stage<T, SPLIT, true, true>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+		else
+			hipLaunchKernelGGL((k_stage<T, SPLIT, false, true>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	} else {
+		if(last)
+			hipLaunchKernelGGL((k_stage<T, SPLIT, true, false>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+		else
+			hipLaunchKernelGGL((k_stage<T, SPLIT, false, false>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	}
+}
+
+void dispatch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	switch(p->precision) {
+	case HELIX_HIP_FP64: launch_stage<double, false>(p, a, last); break;
+	case HELIX_HIP_FP32: launch_stage<float, false>(p, a, last); break;
+	default: launch_stage<float, true>(p, a, last); break;
+	}
+}
+
+// Enqueues stage j (1-based) of the current step.
+void enqueue_stage(helix_hip_plan* p, int j)
+{
+	StageArgs a = p->base;
+	a.scale = p->dt / j;
+	a.cur = (j == 1) ? p->rho : p->t[j - 1];
+	const bool last = (j == p->order);
+	if(!last) {
+		a.out = p->t[j];
+	} else {
+		a.rho = p->rho;
+		a.ntprev = p->order - 1;
+		for(int q = 0; q < a.ntprev; ++q) a.tprev[q] = p->t[q + 1];
+		a.out = (p->order == 1) ? p->t[1] : p->rho;
+		a.rho0_f64 = p->rho0_f64;
+	}
+	dispatch_stage(p, a, last);
+	if(last && p->order == 1) std::swap(p->rho, p->t[1]);
+}
+
+void enqueue_step(helix_hip_plan* p)
+{
+	for(int j = 1; j <= p->order; ++j) enqueue_stage(p, j);
+}
+
+int launch_error(const char* what)
+{
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return hip_fail(e, what);
+	return 0;
+}
+
+int host_to_state(helix_hip_plan* p, const double* state)
+{
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	if(p->precision == HELIX_HIP_FP64) {
+		HX_HIP(hipMemcpy(p->rho, state, p->elems * 16, hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> f(p->elems * 2);
+		for(size_t q = 0; q < p->elems * 2; ++q) f[q] = static_cast<float>(state[q]);
+		HX_HIP(hipMemcpy(p->rho, f.data(), p->elems * 8, hipMemcpyHostToDevice));
+		HX_HIP(hipMemcpy(p->rho0_f64, state, n2 * 16, hipMemcpyHostToDevice));
+	}
+	return 0;
+}
+
+int state_to_host(helix_hip_plan* p, const void* dev, double* out, bool tier0_master)
+{
+	if(p->precision == HELIX_HIP_FP64) {
+		HX_HIP(hipMemcpy(out, dev, p->elems * 16, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> f(p->elems * 2);
+		HX_HIP(hipMemcpy(f.data(), dev, p->elems * 8, hipMemcpyDeviceToHost));
+		for(size_t q = 0; q < p->elems * 2; ++q) out[q] = static_cast<double>(f[q]);
+		if(tier0_master) {
+			const size_t n2 = static_cast<size_t>(p->N) * p->N;
+			HX_HIP(hipMemcpy(out, p->rho0_f64, n2 * 16, hipMemcpyDeviceToHost));
+		}
+	}
+	return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* helix_hip_last_error(void)
+{
+	return g_last_error.c_str();
+}
+
+int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_plan** out)
+{
+	if(!out) return fail(HELIX_HIP_EINVAL, "out is null");
+	*out = nullptr;
+	if(int rc = validate(pr)) return rc;
+	HX_HIP(hipSetDevice(device));
+	helix_hip_plan* p = new(std::nothrow) helix_hip_plan();
+	if(!p) return fail(HELIX_HIP_ENOMEM, "host allocation failed");
+	auto bail = [&](int rc) {
+		helix_hip_plan_destroy(p);
+		return rc;
+	};
+	p->device = device;
+	p->N = pr->n;
+	p->logN = ilog2_exact(pr->n);
+	p->H = pr->n_ado;
+	p->M = pr->n_modes;
+	p->B = pr->n_baths;
+	p->order = pr->order;
+	p->precision = pr->precision;
+	p->dt = pr->dt;
+	p->esize = (pr->precision == HELIX_HIP_FP64) ? 16 : 8;
+	p->elems = static_cast<size_t>(p->H) << (2 * p->logN);
+	for(int b = 0; b < p->B; ++b)
+		if(pr->couplings[b].nflips > 0) p->flip = true;
+	if(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
+
+	int rc = 0;
+	if((rc = plan_alloc(p, &p->rho, p->elems * p->esize))) return bail(rc);
+	const int nt = std::max(1, p->order - 1);
+	for(int q = 1; q <= nt; ++q)
+		if((rc = plan_alloc(p, &p->t[q], p->elems * p->esize))) return bail(rc);
+	if(p->precision != HELIX_HIP_FP64) {
+		void* r0 = nullptr;
+		if((rc = plan_alloc(p, &r0, static_cast<size_t>(p->N) * p->N * 16))) return bail(rc);
+		p->rho0_f64 = static_cast<double*>(r0);
+	}
+	if(p->flip && (rc = plan_alloc(p, &p->buf, p->elems * p->esize * p->B))) return bail(rc);
+
+	StageArgs& a = p->base;
+	std::memset(&a, 0, sizeof(a));
+	const size_t HM = static_cast<size_t>(p->H) * p->M;
+	if((rc = upload(p, pr->up, HM, &a.up)) || (rc = upload(p, pr->down, HM, &a.down)) ||
+		(rc = upload(p, pr->mode_bath, static_cast<size_t>(p->M), &a.mode_bath)))
+		return bail(rc);
+	const double2* d2 = nullptr;
+	if((rc = upload(p, reinterpret_cast<const double2*>(pr->down_comm), HM, &d2))) return bail(rc);
+	a.dcomm = d2;
+	if((rc = upload(p, reinterpret_cast<const double2*>(pr->down_anti), HM, &d2))) return bail(rc);
+	a.danti = d2;
+	if((rc = upload(p, reinterpret_cast<const double2*>(pr->self_coef), static_cast<size_t>(p->H), &d2))) return bail(rc);
+	a.selfc = d2;
+	if((rc = upload(p, reinterpret_cast<const double2*>(pr->xi), static_cast<size_t>(p->B), &d2))) return bail(rc);
+	a.xi = d2;
+	if((rc = upload(p, reinterpret_cast<const double2*>(pr->xi_counter), static_cast<size_t>(p->B), &d2))) return bail(rc);
+	a.xic = d2;
+	std::vector<double> vdiag(static_cast<size_t>(p->B) * p->N);
+	std::vector<int32_t> vfn(p->B), vfm(static_cast<size_t>(p->B) * kMaxFlips, 0);
+	std::vector<double> vfc(static_cast<size_t>(p->B) * kMaxFlips, 0.0);
+	for(int b = 0; b < p->B; ++b) {
+		std::memcpy(&vdiag[static_cast<size_t>(b) * p->N], pr->couplings[b].diag, sizeof(double) * p->N);
+		vfn[b] = pr->couplings[b].nflips;
+		for(int f = 0; f < vfn[b]; ++f) {
+			vfm[b * kMaxFlips + f] = pr->couplings[b].flip_masks[f];
+			vfc[b * kMaxFlips + f] = pr->couplings[b].flip_coefs[f];
+		}
+	}
+	if((rc = upload(p, vdiag.data(), vdiag.size(), &a.vdiag)) || (rc = upload(p, vfn.data(), vfn.size(), &a.vflip_n)) ||
+		(rc = upload(p, vfm.data(), vfm.size(), &a.vflip_mask)) || (rc = upload(p, vfc.data(), vfc.size(), &a.vflip_coef)) ||
+		(rc = upload(p, pr->hamiltonian.diag, static_cast<size_t>(p->N), &a.hdiag)))
+		return bail(rc);
+	a.hflips = pr->hamiltonian.nflips;
+	for(int f = 0; f < a.hflips; ++f) {
+		a.hmask[f] = pr->hamiltonian.flip_masks[f];
+		a.hcoef[f] = pr->hamiltonian.flip_coefs[f];
+	}
+	a.buf = p->buf;
+	a.n_ado = p->H;
+	a.M = p->M;
+	a.B = p->B;
+	a.logN = p->logN;
+	HX_HIP(hipMemsetAsync(p->rho, 0, p->elems * p->esize, p->stream));
+	if(p->rho0_f64) HX_HIP(hipMemsetAsync(p->rho0_f64, 0, static_cast<size_t>(p->N) * p->N * 16, p->stream));
+	HX_HIP(hipStreamSynchronize(p->stream));
+	*out = p;
+	return 0;
+}
+
+void helix_hip_plan_destroy(helix_hip_plan* p)
+{
+	if(!p) return;
+	if(p->stream) (void)hipStreamSynchronize(p->stream);
+	for(hipEvent_t e : p->events) (void)hipEventDestroy(e);
+	for(void* q : p->allocations) (void)hipFree(q);
+	if(p->stream) (void)hipStreamDestroy(p->stream);
+	delete p;
+}
+
+int helix_hip_set_initial(helix_hip_plan* p, const double* rho0)
+{
+	if(!p || !rho0) return fail(HELIX_HIP_EINVAL, "null argument");
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	std::vector<double> state(p->elems * 2, 0.0);
+	std::memcpy(state.data(), rho0, n2 * 16);
+	return host_to_state(p, state.data());
+}
+
+int helix_hip_set_state(helix_hip_plan* p, const double* state)
+{
+	if(!p || !state) return fail(HELIX_HIP_EINVAL, "null argument");
+	return host_to_state(p, state);
+}
+
+int helix_hip_get_state(helix_hip_plan* p, double* state)
+{
+	if(!p || !state) return fail(HELIX_HIP_EINVAL, "null argument");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	return state_to_host(p, p->rho, state, true);
+}
+
+int helix_hip_rhs(helix_hip_plan* p, double* out)
+{
+	if(!p || !out) return fail(HELIX_HIP_EINVAL, "null argument");
+	StageArgs a = p->base;
+	a.scale = 1.0;
+	a.cur = p->rho;
+	a.out = p->t[1];
+	dispatch_stage(p, a, false);
+	if(int rc = launch_error("rhs launch")) return rc;
+	HX_HIP(hipStreamSynchronize(p->stream));
+	return state_to_host(p, p->t[1], out, false);
+}
+
+int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, double* rho_trace, double* diag_trace)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	const bool record = record_every > 0 && (rho_trace || diag_trace);
+	const uint64_t nrec = record ? nsteps / record_every : 0;
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	double* d_rho = nullptr;
+	double* d_diag = nullptr;
+	if(nrec && rho_trace) HX_HIP(hipMalloc(&d_rho, nrec * n2 * 16));
+	if(nrec && diag_trace) HX_HIP(hipMalloc(&d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double)));
+	uint64_t r = 0;
+	for(uint64_t s = 1; s <= nsteps; ++s) {
+		enqueue_step(p);
+		if(nrec && s % record_every == 0 && r < nrec) {
+			const unsigned g = grid_for(n2);
+			double* ro = d_rho ? d_rho + r * n2 * 2 : nullptr;
+			double* dg = d_diag ? d_diag + r * static_cast<size_t>(p->N) : nullptr;
+			if(p->precision == HELIX_HIP_FP64)
+				hipLaunchKernelGGL((k_record<double>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, nullptr, p->logN, ro, dg);
+			else
+				hipLaunchKernelGGL((k_record<float>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, p->rho0_f64, p->logN, ro, dg);
+			++r;
+		}
+		if((s & 63) == 0)
+			if(int rc = launch_error("step launch")) return rc;
+	}
+	if(int rc = launch_error("step launch")) return rc;
+	HX_HIP(hipStreamSynchronize(p->stream));
+	if(d_rho) {
+		HX_HIP(hipMemcpy(rho_trace, d_rho, nrec * n2 * 16, hipMemcpyDeviceToHost));
+		(void)hipFree(d_rho);
+	}
+	if(d_diag) {
+		HX_HIP(hipMemcpy(diag_trace, d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double), hipMemcpyDeviceToHost));
+		(void)hipFree(d_diag);
+	}
+	return 0;
+}
+
+int helix_hip_get_tier0(helix_hip_plan* p, double* out)
+{
+	if(!p || !out) return fail(HELIX_HIP_EINVAL, "null argument");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	if(p->precision == HELIX_HIP_FP64) {
+		HX_HIP(hipMemcpy(out, p->rho, n2 * 16, hipMemcpyDeviceToHost));
+	} else {
+		HX_HIP(hipMemcpy(out, p->rho0_f64, n2 * 16, hipMemcpyDeviceToHost));
+	}
+	return 0;
+}
+
+int helix_hip_time_steps(helix_hip_plan* p, uint64_t nsteps, double* stage_ms, double* total_ms)
+{
+	if(!p || nsteps == 0) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	const size_t need = 2 * static_cast<size_t>(p->order) * nsteps;
+	while(p->events.size() < need) {
+		hipEvent_t e;
+		HX_HIP(hipEventCreate(&e));
+		p->events.push_back(e);
+	}
+	size_t q = 0;
+	for(uint64_t s = 0; s < nsteps; ++s)
+		for(int j = 1; j <= p->order; ++j) {
+			HX_HIP(hipEventRecord(p->events[q++], p->stream));
+			enqueue_stage(p, j);
+			HX_HIP(hipEventRecord(p->events[q++], p->stream));
+		}
+	if(int rc = launch_error("timed step launch")) return rc;
+	HX_HIP(hipEventSynchronize(p->events[q - 1]));
+	std::vector<double> acc(p->order, 0.0);
+	q = 0;
+	for(uint64_t s = 0; s < nsteps; ++s)
+		for(int j = 0; j < p->order; ++j) {
+			float ms = 0.f;
+			HX_HIP(hipEventElapsedTime(&ms, p->events[q], p->events[q + 1]));
+			acc[j] += ms;
+			q += 2;
+		}
+	if(stage_ms)
+		for(int j = 0; j < p->order; ++j) stage_ms[j] = acc[j] / static_cast<double>(nsteps);
+	if(total_ms) {
+		float ms = 0.f;
+		HX_HIP(hipEventElapsedTime(&ms, p->events[0], p->events[q - 1]));
+		*total_ms = ms;
+	}
+	return 0;
+}
+
+int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* launches_per_step, int* uses_flip_coupling,
+	int* uses_dense_hamiltonian)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	if(device_bytes) *device_bytes = p->device_bytes;
+	if(launches_per_step) *launches_per_step = p->order * (p->flip ? 2 : 1);
+	if(uses_flip_coupling) *uses_flip_coupling = p->flip ? 1 : 0;
+	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = 0;
+	return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,136 @@
+// setup_capi.cpp -- C entry points of the host set-up (include/helix_setup.h).
+#include "heom_setup.hpp"
+#include "helix_setup.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+using namespace helix::setup;
+
+struct helix_model {
+	std::unique_ptr<Problem> problem;
+};
+
+namespace {
+thread_local std::string g_setup_error;
+
+int setup_fail(int code, const std::string& msg)
+{
+	g_setup_error = msg;
+	return code;
+}
+}  // namespace
+
+extern "C" {
+
+int helix_model_legacy(int n, double beta, double gamma, double zeta, int pade_terms, int tiers, int order, double dt,
+	int precision, helix_model** out)
+{
+	if(!out) return setup_fail(HELIX_HIP_EINVAL, "out is null");
+	*out = nullptr;
+	try {
+		LegacyModel m;
+		m.N = n;
+		BathSpec bath;
+		bath.beta = beta;
+		bath.gamma = gamma;
+		bath.zeta = zeta;
+		bath.K = pade_terms;
+		bath.coupling = legacy_coupling(m);
+		auto model = std::make_unique<helix_model>();
+		model->problem = std::make_unique<Problem>(legacy_hamiltonian(m), std::vector<BathSpec>{bath}, tiers, order, dt,
+			precision);
+		*out = model.release();
+		return 0;
+	} catch(const std::exception& e) {
+		return setup_fail(HELIX_HIP_EINVAL, e.what());
+	}
+}
+
+int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam, const double* gamma,
+	const double* temperature, int pade_terms, int max_tier, int order, double dt, int precision, helix_model** out)
+{
+	if(!out || !lam || !gamma || !temperature) return setup_fail(HELIX_HIP_EINVAL, "null argument");
+	*out = nullptr;
+	if(spins < 3 || spins > 12) return setup_fail(HELIX_HIP_EINVAL, "spins must be in [3, 12]");
+	try {
+		const std::vector<double> J = spin_glass_couplings(spins, seed);
+		std::vector<BathSpec> baths;
+		for(int s = 0; s < spins; ++s) {
+			BathSpec b;
+			b.beta = 1.0 / temperature[s];
+			b.gamma = gamma[s];
+			b.zeta = 2.0 * lam[s] / gamma[s];
+			b.omega0 = 1.0;
+			b.K = pade_terms;
+			b.coupling = sigma_z(spins, s);
+			baths.push_back(std::move(b));
+		}
+		auto model = std::make_unique<helix_model>();
+		model->problem = std::make_unique<Problem>(spin_glass_hamiltonian(spins, J, h), std::move(baths), max_tier + 1,
+			order, dt, precision);
+		*out = model.release();
+		return 0;
+	} catch(const std::exception& e) {
+		return setup_fail(HELIX_HIP_EINVAL, e.what());
+	}
+}
+
+const helix_hip_problem* helix_model_problem(const helix_model* model)
+{
+	return model ? &model->problem->descriptor() : nullptr;
+}
+
+int helix_model_rows(const helix_model* model, int32_t* rows)
+{
+	if(!model || !rows) return setup_fail(HELIX_HIP_EINVAL, "null argument");
+	const auto& r = model->problem->hierarchy().rows;
+	std::memcpy(rows, r.data(), r.size() * sizeof(int32_t));
+	return 0;
+}
+
+double helix_model_energy_diagonal(const helix_model* model, const double* rho0)
+{
+	return (model && rho0) ? model->problem->energy_diagonal(rho0) : 0.0;
+}
+
+double helix_model_energy_trace(const helix_model* model, const double* rho0)
+{
+	return (model && rho0) ? model->problem->energy_trace(rho0) : 0.0;
+}
+
+void helix_model_free(helix_model* model)
+{
+	delete model;
+}
+
+int helix_setup_pade(int pade_terms, double* poles, double* residues)
+{
+	if(pade_terms <= 0 || !poles || !residues) return setup_fail(HELIX_HIP_EINVAL, "bad arguments");
+	const Pade p = pade_bose(pade_terms);
+	std::memcpy(poles, p.poles.data(), sizeof(double) * pade_terms);
+	std::memcpy(residues, p.residues.data(), sizeof(double) * pade_terms);
+	return 0;
+}
+
+int helix_setup_spin_glass_couplings(int spins, uint64_t seed, double* J)
+{
+	if(spins <= 0 || !J) return setup_fail(HELIX_HIP_EINVAL, "bad arguments");
+	const std::vector<double> v = spin_glass_couplings(spins, seed);
+	std::memcpy(J, v.data(), sizeof(double) * v.size());
+	return 0;
+}
+
+long helix_setup_hierarchy_size(int modes, int tiers)
+{
+	return static_cast<long>(hierarchy_size(modes, tiers));
+}
+
+const char* helix_setup_last_error(void)
+{
+	return g_setup_error.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+#pragma once
+
+#include <helix/examples.h>
+#include <helix/fwd.h>
+#include <helix/types.h>
+#include <helix/version.h>

@@ -1,0 +1,128 @@
+/*
+ * helix_hip.h -- C-ABI between the C++17 host of helix_amd and its gfx950 HIP kernels.
+ *
+ * The ABI replaces the reference's internal hot-path boundary
+ *   initialize()        src/initialize.h:8, src/initialize.cu:342-352
+ *   develop()           src/liouville.h:27, src/liouville.cu:502-591
+ *   getdRhoSparse()     src/liouville.h:33, src/liouville.cu:749-869
+ *   clearLiouvilleStorage() / clearMatrixStorage()   src/liouville.cu:871, src/matrix_storage.cu:32
+ *   ResultExtractor::final_reduced_density()         src/library/result_extractor.cu:35-89
+ * which src/library/legacy_runtime_session.cu:53-99 drives.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - every function returns int: 0 ok, < 0 contract error (HELIX_HIP_E*), > 0 a hipError_t;
+ *     helix_hip_last_error() gives the message of the calling thread's last failure.
+ *   - no exception crosses the ABI; host buffers are caller-owned; device memory and the
+ *     HIP stream are owned by the plan.  One plan per host thread at a time; not re-entrant.
+ *   - complex numbers are interleaved (re, im) doubles; matrices are row-major N x N.
+ *
+ * The HEOM right-hand side evaluated for ADO n (multi-bath form of src/liouville.cu:783-858):
+ *   L(rho)_n = -i[H, rho_n]
+ *            + sum_k (-i) [V_{b(k)}, rho_{up(n,k)}]
+ *            + sum_k ( down_comm[n][k] [V_{b(k)}, rho_{down(n,k)}] + down_anti[n][k] {V_{b(k)}, rho_{down(n,k)}} )
+ *            + self_coef[n] rho_n
+ *            - sum_b xi[b] [V_b, [V_b, rho_n]] - sum_b xi_counter[b] {V_b, [V_b, rho_n]}
+ * and one step is the Taylor-m recurrence rho <- sum_{j=0..m} (dt L)^j / j! rho (src/liouville.cu:537-590).
+ */
+#ifndef HELIX_HIP_H
+#define HELIX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HELIX_HIP_ABI_VERSION 1
+
+/* contract error codes (< 0), cf. src/backend/transpose_backend.h:14-16 */
+#define HELIX_HIP_EINVAL (-1)
+#define HELIX_HIP_EUNSUPPORTED (-2)
+#define HELIX_HIP_ENOMEM (-3)
+#define HELIX_HIP_ESTATE (-4)
+
+enum helix_hip_precision {
+	HELIX_HIP_FP64 = 0,       /* complex128 everywhere (the plain form) */
+	HELIX_HIP_FP32 = 1,       /* complex64 ADOs; tier-0 accumulation in FP64 */
+	HELIX_HIP_SPLIT_FP32 = 2  /* as FP32, with operator values split into two FP32 terms */
+};
+
+/* A "spin-structured" operator: O = diag(d) + sum_f c_f X_f, (X_f)_{i, i^mask_f} = 1.
+ * The legacy V (src/initialize.cu:95-123), the transverse-field H and sigma^z couplings all fit. */
+typedef struct helix_hip_spin_operator {
+	const double* diag;          /* [n] real diagonal */
+	int32_t nflips;              /* number of bit-flip terms (<= 16) */
+	const int32_t* flip_masks;   /* [nflips], 0 < mask < n */
+	const double* flip_coefs;    /* [nflips] real coefficient of each flip */
+} helix_hip_spin_operator;
+
+typedef struct helix_hip_problem {
+	int32_t abi_version;         /* HELIX_HIP_ABI_VERSION */
+	int32_t n;                   /* Hilbert dimension N, a power of two, 8 <= N <= 4096 */
+	int32_t n_ado;               /* hierarchy size H */
+	int32_t n_modes;             /* M exponential modes over all baths */
+	int32_t n_baths;             /* B coupling operators */
+	const int32_t* up;           /* [H][M] index of n+e_k, H when absent  (src/initialize_detail.h:234-269) */
+	const int32_t* down;         /* [H][M] index of n-e_k, H when absent */
+	const int32_t* mode_bath;    /* [M] bath of each mode */
+	const int32_t* tier;         /* [H] sum_k n_k of each ADO (ADO 0 is tier 0) */
+	const double* down_comm;     /* [H][M][2] */
+	const double* down_anti;     /* [H][M][2] */
+	const double* self_coef;     /* [H][2] */
+	const double* xi;            /* [B][2] KXi_b */
+	const double* xi_counter;    /* [B][2] KXiCounter_b */
+	helix_hip_spin_operator hamiltonian;
+	const double* hamiltonian_dense; /* optional [N][N][2]; non-null selects the dense MFMA commutator */
+	const helix_hip_spin_operator* couplings; /* [B] */
+	int32_t precision;           /* enum helix_hip_precision */
+	int32_t order;               /* Taylor order m, 1 <= m <= 8 (ContextOptions::integrationOrder) */
+	double dt;                   /* time step (ContextOptions::timeStep) */
+} helix_hip_problem;
+
+typedef struct helix_hip_plan helix_hip_plan;
+
+/* Uploads tables and operators, allocates the state (rho plus m-1 stage buffers) on `device`.
+ * Replaces initialize() + the first-call initSparse() (src/liouville.cu:685-747). */
+int helix_hip_plan_create(const helix_hip_problem* problem, int device, helix_hip_plan** out);
+
+/* rho_0 <- rho0 (tier 0, [N][N][2]); every other ADO <- 0 (src/initialize.cu:287-307). */
+int helix_hip_set_initial(helix_hip_plan* plan, const double* rho0);
+
+/* Full state in/out, [H][N][N][2] complex128 regardless of the plan precision. */
+int helix_hip_set_state(helix_hip_plan* plan, const double* state);
+int helix_hip_get_state(helix_hip_plan* plan, double* state);
+
+/* out <- L(state), [H][N][N][2] (getdRhoSparse, src/liouville.cu:749-869). */
+int helix_hip_rhs(helix_hip_plan* plan, double* out);
+
+/* Advances nsteps Taylor-m steps (develop() x nsteps, src/library/legacy_runtime_session.cu:79-99).
+ * When record_every > 0, after every record_every-th step the tier-0 block is recorded:
+ *   rho_trace  [nsteps/record_every][N][N][2] (may be NULL)
+ *   diag_trace [nsteps/record_every][N] real diagonal (may be NULL).
+ * Returns after the device finished. */
+int helix_hip_step(helix_hip_plan* plan, uint64_t nsteps, uint64_t record_every, double* rho_trace,
+	double* diag_trace);
+
+/* Tier-0 block (the reduced density rho_S) as [N][N][2] row-major complex128
+ * (ResultExtractor::final_reduced_density, src/library/result_extractor.cu:35-89). */
+int helix_hip_get_tier0(helix_hip_plan* plan, double* out);
+
+/* Device-timed steps: enqueues nsteps steps with HIP events around every stage launch on the plan's
+ * stream; stage_ms[j] (j < order, may be NULL) gets the mean duration of stage j+1 launches and
+ * total_ms the event time of the whole sequence. */
+int helix_hip_time_steps(helix_hip_plan* plan, uint64_t nsteps, double* stage_ms, double* total_ms);
+
+/* Plan facts: bytes of device memory held, launches per step, kernel names. */
+int helix_hip_plan_info(const helix_hip_plan* plan, size_t* device_bytes, int* launches_per_step,
+	int* uses_flip_coupling, int* uses_dense_hamiltonian);
+
+void helix_hip_plan_destroy(helix_hip_plan* plan);
+
+/* Message of the calling thread's last error ("" when none). */
+const char* helix_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

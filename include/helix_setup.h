@@ -1,0 +1,48 @@
+/*
+ * helix_setup.h -- C entry points of the FP64 host set-up (libhelix.so).
+ *
+ * They expose what the reference does inside initialize() (src/initialize.cu:342-352: PSD,
+ * temperature dependence, operators, hierarchy, initial state) as a tabulated problem that
+ * helix_hip_plan_create() consumes.  Used by the Python front (helix_amd/) and by any binding
+ * that cannot link C++; C++ callers use helix::setup::Problem or the public API in <helix/helix.h>.
+ * Return codes follow include/helix_hip.h (0 ok, < 0 contract error).
+ */
+#ifndef HELIX_SETUP_H
+#define HELIX_SETUP_H
+
+#include "helix_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct helix_model helix_model;
+
+/* The reference's compiled legacy spin glass generalised to N = 2^spins (src/parameters.cu:11-39):
+ * one bath (beta, gamma, zeta, K), coupling V = sum S^z + spin flips of spins 0..8, diagonal H. */
+int helix_model_legacy(int n, double beta, double gamma, double zeta, int pade_terms, int tiers, int order, double dt,
+	int precision, helix_model** out);
+
+/* Seeded transverse-field spin glass H = sum_{i<j} J_ij sz_i sz_j + h sum_i sx_i (J_ij ~ N(0,1) from
+ * SplitMix64(seed)) with one sigma^z Drude-Lorentz bath per spin.  lam/gamma/temperature hold one
+ * value per spin; beta = 1/T, zeta = 2 lam / gamma, omega0 = 1.  Tiers 0..max_tier. */
+int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam, const double* gamma,
+	const double* temperature, int pade_terms, int max_tier, int order, double dt, int precision, helix_model** out);
+
+const helix_hip_problem* helix_model_problem(const helix_model* model);
+int helix_model_rows(const helix_model* model, int32_t* rows /* [n_ado][n_modes] */);
+double helix_model_energy_diagonal(const helix_model* model, const double* rho0);
+double helix_model_energy_trace(const helix_model* model, const double* rho0);
+void helix_model_free(helix_model* model);
+
+/* building blocks, exposed for known-answer tests */
+int helix_setup_pade(int pade_terms, double* poles, double* residues);
+int helix_setup_spin_glass_couplings(int spins, uint64_t seed, double* J /* [spins][spins] */);
+long helix_setup_hierarchy_size(int modes, int tiers);
+
+const char* helix_setup_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,161 @@
+"""GPU parity of the HIP plan against the CPU FP64 oracle, through the C-ABI.
+
+Bars: plain FP64 form <= 1e-12 normwise relative (max|a-b| / max|b|) on the RHS, on trajectories of
+rho_S and on <H>(t); the legacy spin glass additionally against the reference's golden FP32 energy
+trace (examples/outputEnergy.txt, abs tol 1e-5); FP32 / split-FP32 forms within max|d rho_S| <= 1e-6
+of the FP64 oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+FP64_TOL = 1e-12
+FORM_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def hx():
+    import helix_amd
+    return helix_amd
+
+
+def _rand_state(H, N, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.standard_normal((H, N, N)) + 1j * rng.standard_normal((H, N, N))
+    return s + np.conj(np.transpose(s, (0, 2, 1)))
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+SPIN_CASES = [
+    dict(spins=3, seed=0, L=4),                                      # 210 ADOs, N=8
+    dict(spins=4, seed=3, L=2),                                      # 45 ADOs, N=16
+    dict(spins=5, seed=7, L=2, bath_params=[(0.05, 0.5, 0.5), (0.2, 1.0, 1.5), (0.5, 2.0, 2.0),
+                                            (0.1, 1.5, 0.7), (0.3, 0.7, 1.1)]),
+    dict(spins=6, seed=1, L=2),                                      # 91 ADOs, N=64
+]
+
+
+@pytest.mark.parametrize("case", SPIN_CASES, ids=lambda c: f"spins{c['spins']}_L{c['L']}")
+def test_rhs_parity_spin_glass_fp64(hx, oracle_mod, case):
+    m = hx.Model.spin_glass(**case)
+    bp = case.get("bath_params")
+    p = oracle_mod.spin_glass_problem(case["spins"], case["seed"], L=case["L"], bath_params=bp)
+    plan = hx.Plan(m)
+    s = _rand_state(m.H, m.N, 11)
+    plan.set_state(s)
+    got = plan.rhs()
+    ref = p.rhs(s)
+    assert _rel(got, ref) <= FP64_TOL
+    assert np.array_equal(plan.get_state(), s)  # set/get round trip is exact in FP64
+
+
+@pytest.mark.parametrize("N", [8, 32, 1024])
+def test_rhs_parity_legacy_fp64(hx, oracle_mod, N):
+    m = hx.Model.legacy(N=N)
+    p = oracle_mod.legacy_problem(N=N)
+    plan = hx.Plan(m)
+    assert plan.info()["flip"]
+    s = _rand_state(m.H, m.N, 5)
+    plan.set_state(s)
+    assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+
+
+def test_cfg0_trajectory_fp64(hx, oracle_mod):
+    """3 spins, L=4 (210 ADOs of 8x8), dt 0.01, 1000 steps: rho_S(t) and <H>(t) every 100 steps."""
+    m = hx.Model.spin_glass(3, 0, L=4, dt=0.01)
+    p = oracle_mod.spin_glass_problem(3, 0, L=4)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    rho_tr, _ = plan.step(1000, record_every=100, record_rho=True)
+    r = p.initial()
+    for k in range(10):
+        r = p.taylor(r, 0.01, 100)
+        assert _rel(rho_tr[k], r[0]) <= FP64_TOL
+        assert abs(m.energy(rho_tr[k]) - p.energy(r)) <= FP64_TOL * max(1.0, abs(p.energy(r)))
+    assert abs(np.trace(rho_tr[-1]) - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 5])
+def test_taylor_orders(hx, oracle_mod, order):
+    m = hx.Model.spin_glass(3, 0, L=3, dt=0.02, order=order)
+    p = oracle_mod.spin_glass_problem(3, 0, L=3)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(25)
+    ref = p.taylor(p.initial(), 0.02, 25, order=order)
+    assert _rel(plan.get_state(), ref) <= FP64_TOL
+
+
+def test_legacy_three_steps_fp64(hx, oracle_mod):
+    m = hx.Model.legacy(N=1024)
+    p = oracle_mod.legacy_problem()
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(3)
+    ref = p.taylor(p.initial(), 0.1, 3)
+    assert _rel(plan.get_state(), ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-5), ("fp32", 1e-5)])
+def test_legacy_golden_energy_trace(hx, oracle_mod, precision, tol):
+    """All 1981 rows of examples/outputEnergy.txt (row r = energy before step r, abs tol 1e-5)."""
+    t, e = oracle_mod.read_energy_trace(os.path.join(GOLDEN, "legacy_outputEnergy.txt"))
+    m = hx.Model.legacy(N=1024, precision=precision)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    _, diag = plan.step(len(e) - 1, record_every=1, record_diag=True)
+    hd = m.tables()["h_diag"]
+    energies = np.concatenate([[m.energy_diagonal(m.initial_rho())], diag @ hd])
+    assert np.max(np.abs(energies - e)) <= tol
+
+
+def test_cfg1_full_size_rhs_fp64(hx, oracle_mod):
+    """6 spins, L=5: 6188 ADOs of 64x64 -- the benchmark workload, one RHS at full size."""
+    m = hx.Model.spin_glass(6, 1, L=5, dt=0.005)
+    p = oracle_mod.spin_glass_problem(6, 1, L=5)
+    plan = hx.Plan(m)
+    s = _rand_state(m.H, m.N, 2)
+    plan.set_state(s)
+    assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+
+
+def test_cfg1_full_size_invariants(hx):
+    """Size-independent properties at full size: unit trace and Hermitian rho_S, tier-0 <H> finite."""
+    m = hx.Model.spin_glass(6, 1, L=5, dt=0.005)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    rho, _ = plan.step(200, record_every=50, record_rho=True)
+    for r in rho:
+        assert abs(np.trace(r) - 1.0) <= 1e-12
+        assert np.max(np.abs(r - r.conj().T)) <= 1e-12
+    assert np.isfinite(m.energy(rho[-1]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split_fp32"])
+def test_precision_forms_cfg0(hx, oracle_mod, precision):
+    m = hx.Model.spin_glass(3, 0, L=4, dt=0.01, precision=precision)
+    p = oracle_mod.spin_glass_problem(3, 0, L=4)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(1000)
+    ref = p.taylor(p.initial(), 0.01, 1000)[0]
+    err = float(np.max(np.abs(plan.tier0() - ref)))
+    print(f"{precision}: max|d rho_S| = {err:.3e}")
+    assert err <= FORM_TOL
+
+
+def test_cpp_example_runs(hx):
+    exe = os.path.join(ROOT, "helix_amd", "helix_example")
+    out = subprocess.run([exe, "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "reduced_density_shape=(1,1024,1024)" in out.stdout
+    tr = float(out.stdout.strip().split("trace=")[1])
+    assert abs(tr - 1.0) <= 1e-5
