@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -258,6 +259,141 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 	}
 }
 
+// LDS-tiled stage for diagonal couplings and N <= 64 (LOGN 3..6).
+// A workgroup owns 4096 elements = 4096/N^2 whole ADO blocks (16 elements per thread).  The blocks are
+// staged once in LDS so every Hamiltonian bit-flip gather (rows i^m and columns j^m) is served from LDS;
+// neighbour ADOs stream from HBM with fully coalesced 256-element rows, mode by mode, so 16 independent
+// loads per thread are in flight.  Coupling diagonals v_b are staged in LDS as well.
+constexpr int kTileEpt = 16;
+constexpr int kTile = kBlock * kTileEpt;
+
+template <typename T, bool SPLIT, bool LAST, int LOGN>
+__global__ __launch_bounds__(kBlock) void k_stage_tile(const StageArgs a)
+{
+	using V2 = typename C2t<T>::type;
+	constexpr int N = 1 << LOGN;
+	constexpr int LOGE = 2 * LOGN;
+	constexpr int E = 1 << LOGE;
+	constexpr int A = kTile / E;  // ADOs per workgroup
+	__shared__ V2 s_blk[kTile];
+	__shared__ double s_v[16 * N];
+
+	const int tid = threadIdx.x;
+	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
+	const int H = a.n_ado, M = a.M, B = a.B;
+	const int ado0 = blockIdx.x * A;
+
+	for(int q = tid; q < B * N; q += kBlock) s_v[q] = a.vdiag[q];
+
+	V2 c[kTileEpt];
+#pragma unroll
+	for(int r = 0; r < kTileEpt; ++r) {
+		const int idx = r * kBlock + tid;
+		const int n = ado0 + (idx >> LOGE);
+		c[r] = (n < H) ? cur[(static_cast<size_t>(n) << LOGE) + (idx & (E - 1))] : V2{T(0), T(0)};
+		s_blk[idx] = c[r];
+	}
+	__syncthreads();
+
+	T ar[kTileEpt], ai[kTileEpt];
+	// -i[H, rho_n] from LDS
+#pragma unroll
+	for(int r = 0; r < kTileEpt; ++r) {
+		const int idx = r * kBlock + tid;
+		const int e = idx & (E - 1);
+		const int base = idx - e;
+		const int i = e >> LOGN, j = e & (N - 1);
+		T hr = T(0), hi = T(0);
+		cmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], 0.0, c[r].x, c[r].y);
+		for(int f = 0; f < a.hflips; ++f) {
+			const int m = a.hmask[f];
+			const V2 x1 = s_blk[base + (e ^ (m << LOGN))];
+			const V2 x2 = s_blk[base + (e ^ m)];
+			cmadd<T, SPLIT>(hr, hi, a.hcoef[f], 0.0, x1.x - x2.x, x1.y - x2.y);
+		}
+		ar[r] = hi;
+		ai[r] = -hr;
+	}
+
+	// neighbours, mode by mode
+	for(int k = 0; k < M; ++k) {
+		const int b = a.mode_bath[k];
+		const double* __restrict__ vb = s_v + b * N;
+#pragma unroll
+		for(int r = 0; r < kTileEpt; ++r) {
+			const int idx = r * kBlock + tid;
+			const int n = __builtin_amdgcn_readfirstlane(ado0 + (idx >> LOGE));
+			if(n >= H) continue;
+			const int e = idx & (E - 1);
+			const int i = e >> LOGN, j = e & (N - 1);
+			const double vi = vb[i], vj = vb[j];
+			const int u = a.up[n * M + k];
+			if(u < H) {
+				const V2 x = cur[(static_cast<size_t>(u) << LOGE) + e];
+				cmadd<T, SPLIT>(ar[r], ai[r], 0.0, -(vi - vj), x.x, x.y);
+			}
+			const int dn = a.down[n * M + k];
+			if(dn < H) {
+				const double2 wc = a.dcomm[n * M + k];
+				const double2 wa = a.danti[n * M + k];
+				const double d = vi - vj, s = vi + vj;
+				const V2 x = cur[(static_cast<size_t>(dn) << LOGE) + e];
+				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x.x, x.y);
+			}
+		}
+	}
+
+	// Sigma, Xi, counter; Taylor epilogue
+	const T sc = static_cast<T>(a.scale);
+#pragma unroll
+	for(int r = 0; r < kTileEpt; ++r) {
+		const int idx = r * kBlock + tid;
+		const int n = __builtin_amdgcn_readfirstlane(ado0 + (idx >> LOGE));
+		if(n >= H) continue;
+		const int e = idx & (E - 1);
+		const int i = e >> LOGN, j = e & (N - 1);
+		const double2 sf = a.selfc[n];
+		double wr = sf.x, wi = sf.y;
+		for(int b = 0; b < B; ++b) {
+			const double vi = s_v[b * N + i], vj = s_v[b * N + j];
+			const double d = vi - vj, s = vi + vj;
+			wr -= a.xi[b].x * d * d + a.xic[b].x * s * d;
+			wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
+		}
+		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c[r].x, c[r].y);
+		const T tr = ar[r] * sc, ti = ai[r] * sc;
+		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
+		V2* __restrict__ out = static_cast<V2*>(a.out);
+		if constexpr(!LAST) {
+			out[g] = V2{tr, ti};
+		} else {
+			if(a.rho0_f64 != nullptr && n == 0) {
+				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
+				double2 rv = r0[e];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					rv.x += static_cast<double>(t.x);
+					rv.y += static_cast<double>(t.y);
+				}
+				rv.x += static_cast<double>(tr);
+				rv.y += static_cast<double>(ti);
+				r0[e] = rv;
+				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
+			} else {
+				V2 rv = static_cast<const V2*>(a.rho)[g];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					rv.x += t.x;
+					rv.y += t.y;
+				}
+				rv.x += tr;
+				rv.y += ti;
+				out[g] = rv;
+			}
+		}
+	}
+}
+
 // tier-0 block -> trace buffers (FP64): full block and/or real diagonal
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, double* rho_out,
@@ -294,6 +430,7 @@ struct helix_hip_plan {
 	int N = 0, logN = 0, H = 0, M = 0, B = 0, order = 4, precision = 0;
 	double dt = 0.0;
 	bool flip = false;
+	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
 	size_t esize = 16;        // bytes per complex element
 	size_t elems = 0;         // H * N * N
 	void* rho = nullptr;
@@ -385,10 +522,29 @@ unsigned grid_for(size_t total)
 	return static_cast<unsigned>(g ? g : 1);
 }
 
+template <typename T, bool SPLIT, int LOGN>
+void launch_tile(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	constexpr int A = kTile >> (2 * LOGN);
+	const unsigned grid = static_cast<unsigned>((p->H + A - 1) / A);
+	if(last)
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	else
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, false, LOGN>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+}
+
 template <typename T, bool SPLIT>
 void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 {
 	const unsigned grid = grid_for(p->elems);
+	if(p->tiled) {
+		switch(p->logN) {
+		case 3: launch_tile<T, SPLIT, 3>(p, a, last); return;
+		case 4: launch_tile<T, SPLIT, 4>(p, a, last); return;
+		case 5: launch_tile<T, SPLIT, 5>(p, a, last); return;
+		default: launch_tile<T, SPLIT, 6>(p, a, last); return;
+		}
+	}
 	if(p->flip) {
 		hipLaunchKernelGGL((k_flip_commutator<T, SPLIT>), dim3(grid_for(p->elems * p->B)), dim3(kBlock), 0, p->stream, a);
 		if(last)
@@ -508,6 +664,10 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	p->elems = static_cast<size_t>(p->H) << (2 * p->logN);
 	for(int b = 0; b < p->B; ++b)
 		if(pr->couplings[b].nflips > 0) p->flip = true;
+	{
+		const char* force = std::getenv("HELIX_HIP_GENERIC_STAGE");
+		p->tiled = !p->flip && p->logN <= 6 && p->B <= 16 && !(force && force[0] == '1');
+	}
 	if(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
 
