@@ -56,6 +56,23 @@ def cpu_baseline(spec, steps: int = 1):
             "sample": f"{steps} Taylor-4 step(s) of the full workload ({p.H} ADOs, N={p.N}) in {sec:.2f} s"}
 
 
+def copy_probe(nbytes: int) -> int:
+    """One hipMemcpy device-to-device of nbytes: a plain copy kernel of known size, so rocprof's
+    FETCH_SIZE / WRITE_SIZE for it give the counter scale (bytes read = bytes written = nbytes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +82,8 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "split_fp32"])
     ap.add_argument("--timed-steps", type=int, default=20, help="steps timed per stage launch with HIP events")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -119,6 +138,8 @@ def main(argv=None):
                      "stage_ms": [float(x) for x in stage_ms]},
         "launches_per_step": info["launches_per_step"],
     }
+    if args.copy_probe_mib:
+        line["copy_probe_bytes"] = copy_probe(args.copy_probe_mib << 20)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(spec)
     print(json.dumps(line), flush=True)

@@ -47,6 +47,14 @@ struct StageArgs {
 	double hcoef[kMaxFlips];
 	int n_ado, M, B, logN;
 	double scale;             // dt / j
+	// sigma^z-structured couplings (every V_b = alpha_b (2 n_{s_b} - 1), no flips): the bit mask of the
+	// spin s_b and alpha_b per bath, and the per-(i XOR j) self-term table
+	//   selfx[x] = -sum_b xi_b (v_b(i) - v_b(j))^2 = -4 sum_{b: x has bit s_b} alpha_b^2 xi_b
+	// (the counter term vanishes because (v_i + v_j)(v_i - v_j) = alpha^2 (1 - 1) = 0).
+	int sz;
+	int sz_mask[16];
+	double sz_alpha[16];
+	const double2* selfx;     // [N]
 };
 
 }  // namespace helix_amd

@@ -260,110 +260,186 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 }
 
 // LDS-tiled stage for diagonal couplings and N <= 64 (LOGN 3..6).
-// A workgroup owns 4096 elements = 4096/N^2 whole ADO blocks (16 elements per thread).  The blocks are
-// staged once in LDS so every Hamiltonian bit-flip gather (rows i^m and columns j^m) is served from LDS;
-// neighbour ADOs stream from HBM with fully coalesced 256-element rows, mode by mode, so 16 independent
-// loads per thread are in flight.  Coupling diagonals v_b are staged in LDS as well.
-constexpr int kTileEpt = 16;
-constexpr int kTile = kBlock * kTileEpt;
+// A workgroup owns a tile of 4096 elements = A = 4096/N^2 whole ADO blocks; each ADO is handled by
+// THREADS/A threads with 4096/THREADS elements per thread (element e = lane_in_ado + (THREADS/A) r, so
+// every global access is a contiguous 16-byte-per-lane row).  The tile's own blocks are staged once in
+// LDS, so every Hamiltonian bit-flip gather (rows i^m, columns j^m) is served from LDS.  The per-ADO
+// neighbour table (n+e_k, n-e_k) is copied to LDS and walked entry by entry; each valid neighbour issues
+// all of its loads before any arithmetic.  Workgroups are renumbered so each XCD (round-robin dispatch
+// over 8 XCDs) sweeps one contiguous range of ADOs: the near neighbours in lexicographic order
+// (n +- e_k for the trailing modes) then hit that XCD's L2.
+// SZ: every coupling is alpha_b (2 n_{s_b} - 1): commutator / anticommutator weights come from bits
+// (d_b = v_i - v_j and s_b = v_i + v_j, one of them zero) and the Sigma + Xi weight from selfx[i^j].
+constexpr int kTile = 4096;  // elements per workgroup
+constexpr int kTileMaxModes = 16;
+constexpr int kXcds = 8;
 
-template <typename T, bool SPLIT, bool LAST, int LOGN>
-__global__ __launch_bounds__(kBlock) void k_stage_tile(const StageArgs a)
+// acc += w * x for a real weight w (two FMAs; cmadd with a zero imaginary part would spend four)
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void rmadd(T& ar, T& ai, double w, T xr, T xi)
+{
+	if constexpr(SPLIT) {
+		const float h = static_cast<float>(w);
+		const float l = static_cast<float>(w - static_cast<double>(h));
+		ar += l * xr;
+		ai += l * xi;
+		ar += h * xr;
+		ai += h * xi;
+	} else {
+		const T r = static_cast<T>(w);
+		ar += r * xr;
+		ai += r * xi;
+	}
+}
+
+template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
+k_stage_tile(const StageArgs a, int ngroups)
 {
 	using V2 = typename C2t<T>::type;
+	constexpr int EPT = kTile / THREADS;
 	constexpr int N = 1 << LOGN;
 	constexpr int LOGE = 2 * LOGN;
 	constexpr int E = 1 << LOGE;
-	constexpr int A = kTile / E;  // ADOs per workgroup
+	constexpr int A = kTile / E;    // ADOs per workgroup
+	constexpr int TA = THREADS / A; // threads per ADO
 	__shared__ V2 s_blk[kTile];
-	__shared__ double s_v[16 * N];
+	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
+	__shared__ double2 s_selfx[SZ ? N : 1];
+	__shared__ int s_nb[A][2 * kTileMaxModes];
+
+	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
+	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
+	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+	if(group >= ngroups) return;
 
 	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
 	const int H = a.n_ado, M = a.M, B = a.B;
-	const int ado0 = blockIdx.x * A;
+	const int slot = tid / TA;
+	const int lt = tid - slot * TA;
+	const int n = group * A + slot;  // this thread's ADO (uniform over TA threads)
+	const bool live = n < H;
 
-	for(int q = tid; q < B * N; q += kBlock) s_v[q] = a.vdiag[q];
-
-	V2 c[kTileEpt];
+	if constexpr(SZ) {
+		for(int q = tid; q < N; q += THREADS) s_selfx[q] = a.selfx[q];
+	} else {
+		for(int q = tid; q < B * N; q += THREADS) s_v[q] = a.vdiag[q];
+	}
+	for(int q = tid; q < A * 2 * M; q += THREADS) {
+		const int s = q / (2 * M), k = q - s * (2 * M);
+		const int nn = group * A + s;
+		int idx = -1;
+		if(nn < H) {
+			const int v = (k < M) ? a.up[nn * M + k] : a.down[nn * M + (k - M)];
+			idx = (v < H) ? v : -1;
+		}
+		s_nb[s][k] = idx;
+	}
 #pragma unroll
-	for(int r = 0; r < kTileEpt; ++r) {
-		const int idx = r * kBlock + tid;
-		const int n = ado0 + (idx >> LOGE);
-		c[r] = (n < H) ? cur[(static_cast<size_t>(n) << LOGE) + (idx & (E - 1))] : V2{T(0), T(0)};
-		s_blk[idx] = c[r];
+	for(int r = 0; r < EPT; ++r) {
+		const int e = lt + TA * r;
+		s_blk[slot * E + e] = live ? cur[(static_cast<size_t>(n) << LOGE) + e] : V2{T(0), T(0)};
 	}
 	__syncthreads();
+	if(!live) return;
 
-	T ar[kTileEpt], ai[kTileEpt];
+	T ar[EPT], ai[EPT];
+	const V2* __restrict__ blk = s_blk + slot * E;
 	// -i[H, rho_n] from LDS
 #pragma unroll
-	for(int r = 0; r < kTileEpt; ++r) {
-		const int idx = r * kBlock + tid;
-		const int e = idx & (E - 1);
-		const int base = idx - e;
+	for(int r = 0; r < EPT; ++r) {
+		const int e = lt + TA * r;
 		const int i = e >> LOGN, j = e & (N - 1);
+		const V2 c = blk[e];
 		T hr = T(0), hi = T(0);
-		cmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], 0.0, c[r].x, c[r].y);
+		rmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], c.x, c.y);
 		for(int f = 0; f < a.hflips; ++f) {
 			const int m = a.hmask[f];
-			const V2 x1 = s_blk[base + (e ^ (m << LOGN))];
-			const V2 x2 = s_blk[base + (e ^ m)];
-			cmadd<T, SPLIT>(hr, hi, a.hcoef[f], 0.0, x1.x - x2.x, x1.y - x2.y);
+			const V2 x1 = blk[e ^ (m << LOGN)];
+			const V2 x2 = blk[e ^ m];
+			rmadd<T, SPLIT>(hr, hi, a.hcoef[f], x1.x - x2.x, x1.y - x2.y);
 		}
 		ar[r] = hi;
 		ai[r] = -hr;
 	}
 
-	// neighbours, mode by mode
-	for(int k = 0; k < M; ++k) {
+	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}}
+	for(int q = 0; q < 2 * M; ++q) {
+		const int idx = s_nb[slot][q];
+		if(idx < 0) continue;
+		const bool down = q >= M;
+		const int k = down ? q - M : q;
 		const int b = a.mode_bath[k];
-		const double* __restrict__ vb = s_v + b * N;
+		const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
+		V2 x[EPT];
 #pragma unroll
-		for(int r = 0; r < kTileEpt; ++r) {
-			const int idx = r * kBlock + tid;
-			const int n = __builtin_amdgcn_readfirstlane(ado0 + (idx >> LOGE));
-			if(n >= H) continue;
-			const int e = idx & (E - 1);
-			const int i = e >> LOGN, j = e & (N - 1);
-			const double vi = vb[i], vj = vb[j];
-			const int u = a.up[n * M + k];
-			if(u < H) {
-				const V2 x = cur[(static_cast<size_t>(u) << LOGE) + e];
-				cmadd<T, SPLIT>(ar[r], ai[r], 0.0, -(vi - vj), x.x, x.y);
+		for(int r = 0; r < EPT; ++r) x[r] = src[lt + TA * r];
+		if constexpr(SZ) {
+			// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
+			// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
+			// Up:   (-i) d x.   Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
+			const int mask = a.sz_mask[b];
+			const double two_alpha = 2.0 * a.sz_alpha[b];
+			double wcomm = -1.0, wanti = 0.0;  // up: weight -i on the commutator part
+			if(down) {
+				wcomm = a.dcomm[n * M + k].y;
+				wanti = a.danti[n * M + k].x;
 			}
-			const int dn = a.down[n * M + k];
-			if(dn < H) {
-				const double2 wc = a.dcomm[n * M + k];
-				const double2 wa = a.danti[n * M + k];
+#pragma unroll
+			for(int r = 0; r < EPT; ++r) {
+				const int e = lt + TA * r;
+				const int bi = (e >> LOGN) & mask, bj = e & mask;
+				const bool differ = bi != bj;
+				const double sg = bi ? two_alpha : -two_alpha;
+				const double w = sg * (differ ? wcomm : wanti);
+				// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
+				const T u = differ ? -x[r].y : x[r].x;
+				const T v = differ ? x[r].x : x[r].y;
+				rmadd<T, SPLIT>(ar[r], ai[r], w, u, v);
+			}
+		} else {
+			double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
+			if(down) {
+				wc = a.dcomm[n * M + k];
+				wa = a.danti[n * M + k];
+			}
+			const double* __restrict__ vb = s_v + b * N;
+#pragma unroll
+			for(int r = 0; r < EPT; ++r) {
+				const int e = lt + TA * r;
+				const double vi = vb[e >> LOGN], vj = vb[e & (N - 1)];
 				const double d = vi - vj, s = vi + vj;
-				const V2 x = cur[(static_cast<size_t>(dn) << LOGE) + e];
-				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x.x, x.y);
+				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
 			}
 		}
 	}
 
 	// Sigma, Xi, counter; Taylor epilogue
 	const T sc = static_cast<T>(a.scale);
+	const double2 sf = a.selfc[n];
+	V2* __restrict__ out = static_cast<V2*>(a.out);
 #pragma unroll
-	for(int r = 0; r < kTileEpt; ++r) {
-		const int idx = r * kBlock + tid;
-		const int n = __builtin_amdgcn_readfirstlane(ado0 + (idx >> LOGE));
-		if(n >= H) continue;
-		const int e = idx & (E - 1);
+	for(int r = 0; r < EPT; ++r) {
+		const int e = lt + TA * r;
 		const int i = e >> LOGN, j = e & (N - 1);
-		const double2 sf = a.selfc[n];
 		double wr = sf.x, wi = sf.y;
-		for(int b = 0; b < B; ++b) {
-			const double vi = s_v[b * N + i], vj = s_v[b * N + j];
-			const double d = vi - vj, s = vi + vj;
-			wr -= a.xi[b].x * d * d + a.xic[b].x * s * d;
-			wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
+		if constexpr(SZ) {
+			const double2 x = s_selfx[i ^ j];
+			wr += x.x;
+			wi += x.y;
+		} else {
+			for(int b = 0; b < B; ++b) {
+				const double vi = s_v[b * N + i], vj = s_v[b * N + j];
+				const double d = vi - vj, s = vi + vj;
+				wr -= a.xi[b].x * d * d + a.xic[b].x * s * d;
+				wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
+			}
 		}
-		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c[r].x, c[r].y);
+		const V2 c = blk[e];
+		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c.x, c.y);
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
-		V2* __restrict__ out = static_cast<V2*>(a.out);
 		if constexpr(!LAST) {
 			out[g] = V2{tr, ti};
 		} else {
@@ -522,15 +598,37 @@ unsigned grid_for(size_t total)
 	return static_cast<unsigned>(g ? g : 1);
 }
 
+template <typename T, bool SPLIT, int LOGN, int THREADS, bool SZ>
+void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	constexpr int A = kTile >> (2 * LOGN);
+	const int ngroups = (p->H + A - 1) / A;
+	const unsigned grid = static_cast<unsigned>(((ngroups + kXcds - 1) / kXcds) * kXcds);
+	if(last)
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN, THREADS, SZ>), dim3(grid), dim3(THREADS), 0, p->stream,
+			a, ngroups);
+	else
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, false, LOGN, THREADS, SZ>), dim3(grid), dim3(THREADS), 0,
+			p->stream, a, ngroups);
+}
+
+// 512 threads for complex128 tiles (64 KiB LDS, 4 waves/SIMD), 1024 for complex64 tiles (32 KiB, 8 waves/SIMD)
+template <typename T, bool SPLIT, int LOGN, bool SZ>
+void launch_tile_sz(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	if constexpr(sizeof(T) == 8)
+		launch_tile_v<T, SPLIT, LOGN, 512, SZ>(p, a, last);
+	else
+		launch_tile_v<T, SPLIT, LOGN, 1024, SZ>(p, a, last);
+}
+
 template <typename T, bool SPLIT, int LOGN>
 void launch_tile(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	constexpr int A = kTile >> (2 * LOGN);
-	const unsigned grid = static_cast<unsigned>((p->H + A - 1) / A);
-	if(last)
-		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	if(a.sz)
+		launch_tile_sz<T, SPLIT, LOGN, true>(p, a, last);
 	else
-		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, false, LOGN>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+		launch_tile_sz<T, SPLIT, LOGN, false>(p, a, last);
 }
 
 template <typename T, bool SPLIT>
@@ -666,7 +764,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		if(pr->couplings[b].nflips > 0) p->flip = true;
 	{
 		const char* force = std::getenv("HELIX_HIP_GENERIC_STAGE");
-		p->tiled = !p->flip && p->logN <= 6 && p->B <= 16 && !(force && force[0] == '1');
+		p->tiled = !p->flip && p->logN <= 6 && p->B <= 16 && p->M <= kTileMaxModes && !(force && force[0] == '1');
 	}
 	if(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
@@ -719,6 +817,38 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	for(int f = 0; f < a.hflips; ++f) {
 		a.hmask[f] = pr->hamiltonian.flip_masks[f];
 		a.hcoef[f] = pr->hamiltonian.flip_coefs[f];
+	}
+	// sigma^z structure: every coupling diagonal equals alpha_b (2 n_s - 1) for one spin s, no flips
+	a.sz = (p->B <= 16) ? 1 : 0;
+	for(int b = 0; b < p->B && a.sz; ++b) {
+		const helix_hip_spin_operator& op = pr->couplings[b];
+		int found = -1;
+		for(int s = 0; s < p->logN && found < 0 && op.nflips == 0; ++s) {
+			const double alpha = op.diag[1 << s];
+			bool ok = alpha != 0.0;
+			for(int i = 0; i < p->N && ok; ++i) ok = op.diag[i] == (((i >> s) & 1) ? alpha : -alpha);
+			if(ok) found = s;
+		}
+		if(found < 0) {
+			a.sz = 0;
+		} else {
+			a.sz_mask[b] = 1 << found;
+			a.sz_alpha[b] = op.diag[1 << found];
+		}
+	}
+	for(size_t q = 0; q < HM && a.sz; ++q)
+		if(pr->down_comm[2 * q] != 0.0 || pr->down_anti[2 * q + 1] != 0.0) a.sz = 0;
+	if(a.sz) {
+		std::vector<double> selfx(static_cast<size_t>(p->N) * 2, 0.0);
+		for(int x = 0; x < p->N; ++x)
+			for(int b = 0; b < p->B; ++b)
+				if(x & a.sz_mask[b]) {
+					const double d2 = 4.0 * a.sz_alpha[b] * a.sz_alpha[b];
+					selfx[2 * x] -= pr->xi[2 * b] * d2;
+					selfx[2 * x + 1] -= pr->xi[2 * b + 1] * d2;
+				}
+		if((rc = upload(p, reinterpret_cast<const double2*>(selfx.data()), static_cast<size_t>(p->N), &d2))) return bail(rc);
+		a.selfx = d2;
 	}
 	a.buf = p->buf;
 	a.n_ado = p->H;
