@@ -301,8 +301,10 @@ k_stage_tile(const StageArgs a, int ngroups)
 	constexpr int N = 1 << LOGN;
 	constexpr int LOGE = 2 * LOGN;
 	constexpr int E = 1 << LOGE;
-	constexpr int A = kTile / E;    // ADOs per workgroup
-	constexpr int TA = THREADS / A; // threads per ADO
+	constexpr int TPA = (E > kTile) ? E / kTile : 1;  // tiles per ADO (N > 64: row tiles of one block)
+	constexpr int A = (E >= kTile) ? 1 : kTile / E;   // ADOs per workgroup (N < 64)
+	constexpr int TE = (E >= kTile) ? kTile : E;      // elements of one ADO held by the tile
+	constexpr int TA = THREADS / A;                   // threads per ADO
 	__shared__ V2 s_blk[kTile];
 	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
 	__shared__ double2 s_selfx[SZ ? N : 1];
@@ -317,8 +319,9 @@ k_stage_tile(const StageArgs a, int ngroups)
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
 	const int H = a.n_ado, M = a.M, B = a.B;
 	const int slot = tid / TA;
+	const int ebase = (group % TPA) * kTile;  // first element of this row tile
 	const int lt = tid - slot * TA;
-	const int n = group * A + slot;  // this thread's ADO (uniform over TA threads)
+	const int n = (group / TPA) * A + slot;  // this thread's ADO (uniform over TA threads)
 	const bool live = n < H;
 
 	if constexpr(SZ) {
@@ -328,7 +331,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 	}
 	for(int q = tid; q < A * 2 * M; q += THREADS) {
 		const int s = q / (2 * M), k = q - s * (2 * M);
-		const int nn = group * A + s;
+		const int nn = (group / TPA) * A + s;
 		int idx = -1;
 		if(nn < H) {
 			const int v = (k < M) ? a.up[nn * M + k] : a.down[nn * M + (k - M)];
@@ -338,26 +341,29 @@ k_stage_tile(const StageArgs a, int ngroups)
 	}
 #pragma unroll
 	for(int r = 0; r < EPT; ++r) {
-		const int e = lt + TA * r;
-		s_blk[slot * E + e] = live ? cur[(static_cast<size_t>(n) << LOGE) + e] : V2{T(0), T(0)};
+		const int le = lt + TA * r;
+		s_blk[slot * TE + le] = live ? cur[(static_cast<size_t>(n) << LOGE) + ebase + le] : V2{T(0), T(0)};
 	}
 	__syncthreads();
 	if(!live) return;
 
 	T ar[EPT], ai[EPT];
-	const V2* __restrict__ blk = s_blk + slot * E;
+	const V2* __restrict__ blk = s_blk + slot * TE;
+	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);
 	// -i[H, rho_n] from LDS
 #pragma unroll
 	for(int r = 0; r < EPT; ++r) {
-		const int e = lt + TA * r;
+		const int le = lt + TA * r;
+		const int e = ebase + le;
 		const int i = e >> LOGN, j = e & (N - 1);
-		const V2 c = blk[e];
+		const V2 c = blk[le];
 		T hr = T(0), hi = T(0);
 		rmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], c.x, c.y);
 		for(int f = 0; f < a.hflips; ++f) {
 			const int m = a.hmask[f];
-			const V2 x1 = blk[e ^ (m << LOGN)];
-			const V2 x2 = blk[e ^ m];
+			const int rf = m << LOGN;  // row flip stays in the tile unless it crosses tiles
+			const V2 x1 = (TPA == 1 || rf < kTile) ? blk[le ^ rf] : own[e ^ rf];
+			const V2 x2 = blk[le ^ m];
 			rmadd<T, SPLIT>(hr, hi, a.hcoef[f], x1.x - x2.x, x1.y - x2.y);
 		}
 		ar[r] = hi;
@@ -374,7 +380,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 		const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
 		V2 x[EPT];
 #pragma unroll
-		for(int r = 0; r < EPT; ++r) x[r] = src[lt + TA * r];
+		for(int r = 0; r < EPT; ++r) x[r] = src[ebase + lt + TA * r];
 		if constexpr(SZ) {
 			// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
 			// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
@@ -388,7 +394,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 			}
 #pragma unroll
 			for(int r = 0; r < EPT; ++r) {
-				const int e = lt + TA * r;
+				const int e = ebase + lt + TA * r;
 				const int bi = (e >> LOGN) & mask, bj = e & mask;
 				const bool differ = bi != bj;
 				const double sg = bi ? two_alpha : -two_alpha;
@@ -407,7 +413,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 			const double* __restrict__ vb = s_v + b * N;
 #pragma unroll
 			for(int r = 0; r < EPT; ++r) {
-				const int e = lt + TA * r;
+				const int e = ebase + lt + TA * r;
 				const double vi = vb[e >> LOGN], vj = vb[e & (N - 1)];
 				const double d = vi - vj, s = vi + vj;
 				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
@@ -415,13 +421,15 @@ k_stage_tile(const StageArgs a, int ngroups)
 		}
 	}
 
-	// Sigma, Xi, counter; Taylor epilogue
+	// Sigma, Xi, counter; Taylor epilogue.  In the last stage t_{m-1} is this tile's own LDS block
+	// (tprev[ntprev-1] == cur), and for order 1 so is rho.
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
-#pragma unroll
+#pragma unroll 2
 	for(int r = 0; r < EPT; ++r) {
-		const int e = lt + TA * r;
+		const int le = lt + TA * r;
+		const int e = ebase + le;
 		const int i = e >> LOGN, j = e & (N - 1);
 		double wr = sf.x, wi = sf.y;
 		if constexpr(SZ) {
@@ -436,31 +444,40 @@ k_stage_tile(const StageArgs a, int ngroups)
 				wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
 			}
 		}
-		const V2 c = blk[e];
+		const V2 c = blk[le];
 		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c.x, c.y);
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
 		if constexpr(!LAST) {
 			out[g] = V2{tr, ti};
 		} else {
+			const int nt = a.ntprev;
 			if(a.rho0_f64 != nullptr && n == 0) {
 				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
 				double2 rv = r0[e];
-				for(int q = 0; q < a.ntprev; ++q) {
+				for(int q = 0; q + 1 < nt; ++q) {
 					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
 					rv.x += static_cast<double>(t.x);
 					rv.y += static_cast<double>(t.y);
+				}
+				if(nt > 0) {
+					rv.x += static_cast<double>(c.x);
+					rv.y += static_cast<double>(c.y);
 				}
 				rv.x += static_cast<double>(tr);
 				rv.y += static_cast<double>(ti);
 				r0[e] = rv;
 				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
 			} else {
-				V2 rv = static_cast<const V2*>(a.rho)[g];
-				for(int q = 0; q < a.ntprev; ++q) {
+				V2 rv = (nt == 0) ? c : static_cast<const V2*>(a.rho)[g];
+				for(int q = 0; q + 1 < nt; ++q) {
 					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
 					rv.x += t.x;
 					rv.y += t.y;
+				}
+				if(nt > 0) {
+					rv.x += c.x;
+					rv.y += c.y;
 				}
 				rv.x += tr;
 				rv.y += ti;
@@ -601,8 +618,10 @@ unsigned grid_for(size_t total)
 template <typename T, bool SPLIT, int LOGN, int THREADS, bool SZ>
 void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	constexpr int A = kTile >> (2 * LOGN);
-	const int ngroups = (p->H + A - 1) / A;
+	constexpr int E = 1 << (2 * LOGN);
+	constexpr int TPA = (E > kTile) ? E / kTile : 1;
+	constexpr int A = (E >= kTile) ? 1 : kTile / E;
+	const int ngroups = ((p->H + A - 1) / A) * TPA;
 	const unsigned grid = static_cast<unsigned>(((ngroups + kXcds - 1) / kXcds) * kXcds);
 	if(last)
 		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN, THREADS, SZ>), dim3(grid), dim3(THREADS), 0, p->stream,
@@ -625,10 +644,12 @@ void launch_tile_sz(helix_hip_plan* p, const StageArgs& a, bool last)
 template <typename T, bool SPLIT, int LOGN>
 void launch_tile(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	if(a.sz)
+	if(a.sz) {
 		launch_tile_sz<T, SPLIT, LOGN, true>(p, a, last);
-	else
-		launch_tile_sz<T, SPLIT, LOGN, false>(p, a, last);
+	} else {
+		// general diagonal couplings keep their B x N diagonals in LDS: N <= 64 only
+		if constexpr(LOGN <= 6) launch_tile_sz<T, SPLIT, LOGN, false>(p, a, last);
+	}
 }
 
 template <typename T, bool SPLIT>
@@ -640,7 +661,11 @@ void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 		case 3: launch_tile<T, SPLIT, 3>(p, a, last); return;
 		case 4: launch_tile<T, SPLIT, 4>(p, a, last); return;
 		case 5: launch_tile<T, SPLIT, 5>(p, a, last); return;
-		default: launch_tile<T, SPLIT, 6>(p, a, last); return;
+		case 6: launch_tile<T, SPLIT, 6>(p, a, last); return;
+		case 7: launch_tile<T, SPLIT, 7>(p, a, last); return;
+		case 8: launch_tile<T, SPLIT, 8>(p, a, last); return;
+		case 9: launch_tile<T, SPLIT, 9>(p, a, last); return;
+		default: launch_tile<T, SPLIT, 10>(p, a, last); return;
 		}
 	}
 	if(p->flip) {
@@ -764,7 +789,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		if(pr->couplings[b].nflips > 0) p->flip = true;
 	{
 		const char* force = std::getenv("HELIX_HIP_GENERIC_STAGE");
-		p->tiled = !p->flip && p->logN <= 6 && p->B <= 16 && p->M <= kTileMaxModes && !(force && force[0] == '1');
+		p->tiled = !p->flip && p->B <= 16 && p->M <= kTileMaxModes && !(force && force[0] == '1');
 	}
 	if(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
@@ -850,6 +875,8 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		if((rc = upload(p, reinterpret_cast<const double2*>(selfx.data()), static_cast<size_t>(p->N), &d2))) return bail(rc);
 		a.selfx = d2;
 	}
+	if(p->logN > 6 && !a.sz) p->tiled = false;  // see launch_tile
+	if(p->logN > 10) p->tiled = false;
 	a.buf = p->buf;
 	a.n_ado = p->H;
 	a.M = p->M;

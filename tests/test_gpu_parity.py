@@ -60,6 +60,40 @@ def test_rhs_parity_spin_glass_fp64(hx, oracle_mod, case, kernel, monkeypatch):
     assert np.array_equal(plan.get_state(), s)  # set/get round trip is exact in FP64
 
 
+@pytest.mark.parametrize("kernel", ["tiled", "generic"])
+@pytest.mark.parametrize("spins,L", [(7, 2), (8, 1), (9, 1), (10, 1)])
+def test_rhs_parity_large_n_fp64(hx, oracle_mod, spins, L, kernel, monkeypatch):
+    """N = 128..1024: row-tiled stage (far row flips from L2) and the untiled kernel."""
+    if kernel == "generic":
+        monkeypatch.setenv("HELIX_HIP_GENERIC_STAGE", "1")
+    m = hx.Model.spin_glass(spins, seed=2, L=L, lam=0.3, gamma=0.7, T=1.3)
+    p = oracle_mod.spin_glass_problem(spins, 2, L=L, lam=0.3, gamma=0.7, T=1.3)
+    plan = hx.Plan(m)
+    s = _rand_state(m.H, m.N, 21)
+    plan.set_state(s)
+    assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+
+
+def test_precision_form_n256_full_size(hx):
+    """Precision form at its full size (8 spins, L=3: 969 ADOs of 256x256, dt 0.005, 1000 steps):
+    complex64 and split-FP32 against the plain FP64 GPU form, whose RHS is pinned to the oracle above."""
+    ref_m = hx.Model.spin_glass(8, 2, L=3, dt=0.005, precision="fp64")
+    ref = hx.Plan(ref_m)
+    ref.set_initial()
+    ref.step(1000)
+    r64 = ref.tier0()
+    ref.close()
+    for prec in ("fp32", "split_fp32"):
+        m = hx.Model.spin_glass(8, 2, L=3, dt=0.005, precision=prec)
+        plan = hx.Plan(m)
+        plan.set_initial()
+        plan.step(1000)
+        err = float(np.max(np.abs(plan.tier0() - r64)))
+        print(f"{prec}: N=256 969 ADOs 1000 steps max|d rho_S| = {err:.3e}")
+        assert err <= FORM_TOL
+        plan.close()
+
+
 @pytest.mark.parametrize("N", [8, 32, 1024])
 def test_rhs_parity_legacy_fp64(hx, oracle_mod, N):
     m = hx.Model.legacy(N=N)
