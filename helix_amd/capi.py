@@ -25,7 +25,8 @@ HIP_SYMBOLS = [
     "helix_hip_last_error",
 ]
 SETUP_SYMBOLS = [
-    "helix_model_legacy", "helix_model_spin_glass", "helix_model_problem", "helix_model_rows",
+    "helix_model_legacy", "helix_model_spin_glass", "helix_model_use_dense_hamiltonian", "helix_model_problem",
+    "helix_model_rows",
     "helix_model_energy_diagonal", "helix_model_energy_trace", "helix_model_free", "helix_setup_pade",
     "helix_setup_spin_glass_couplings", "helix_setup_hierarchy_size", "helix_setup_last_error",
 ]
@@ -97,6 +98,7 @@ def host():
                                          C.c_double, C.c_int, C.POINTER(vp)]
         L.helix_model_spin_glass.argtypes = [C.c_int, C.c_uint64, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int,
                                              C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+        L.helix_model_use_dense_hamiltonian.argtypes = [vp, C.c_int]
         L.helix_model_problem.argtypes = [vp]
         L.helix_model_problem.restype = C.POINTER(Problem)
         L.helix_model_rows.argtypes = [vp, _ip]
@@ -159,6 +161,11 @@ class Model:
                                            PRECISIONS[precision], C.byref(hdl))
         _check(rc, "helix_model_spin_glass", host())
         return cls(hdl, 1 << spins)
+
+    def use_dense_hamiltonian(self, enable: bool = True):
+        """Route -i[H, rho] through the dense MFMA commutator (H materialised as N x N)."""
+        _check(host().helix_model_use_dense_hamiltonian(self._h, int(enable)), "use_dense_hamiltonian", host())
+        return self
 
     def problem_ptr(self):
         return host().helix_model_problem(self._h)

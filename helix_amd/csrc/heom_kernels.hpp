@@ -55,6 +55,11 @@ struct StageArgs {
 	int sz_mask[16];
 	double sz_alpha[16];
 	const double2* selfx;     // [N]
+	// dense Hamiltonian (MFMA commutator path): [N][N] complex in the state precision; for the
+	// split-FP32 form hdense holds the FP32 high part and hdense_lo the FP32 remainder
+	const void* hdense;
+	const void* hdense_lo;
+	int hreal;                // every Im H_ij == 0: the imaginary-H products are skipped
 };
 
 }  // namespace helix_amd

@@ -487,6 +487,240 @@ k_stage_tile(const StageArgs a, int ngroups)
 	}
 }
 
+// Dense-Hamiltonian stage for N in {16, 32, 64}: one ADO per 256-thread workgroup.
+// rho_n and H sit in LDS with a padded row stride (N+1) so the 16 rows an MFMA operand read touches
+// fall in different banks.  G = H rho - rho H is computed as real 16x16x4 MFMA products of the real
+// and imaginary planes (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32; the split-FP32 form adds the
+// products with the FP32 remainder of H), each wave owning whole 16x16 output tiles.  The result
+// -i G + (Sigma + Xi) rho replaces rho_n in LDS; the neighbour terms and the Taylor epilogue follow as
+// in the tiled kernel.  MFMA operand layout (16x16x4, measured on gfx950): A[i][k] and B[k][j] from
+// lane l with i (or j) = l % 16, k = l / 16; D[i][j] in register v of lane l with j = l % 16 and
+// i = 4 (l / 16) + v for f32 but i = l / 16 + 4 v for f64.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+	using acc = double __attribute__((ext_vector_type(4)));
+	__device__ static acc mma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+	__device__ static int row(int lk, int v) { return lk + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+	using acc = float __attribute__((ext_vector_type(4)));
+	__device__ static acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+	__device__ static int row(int lk, int v) { return 4 * lk + v; }
+};
+
+constexpr int kDenseThreads = 256;
+
+template <typename T, bool SPLIT, bool LAST, int LOGN, bool SZ>
+__global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a)
+{
+	using V2 = typename C2t<T>::type;
+	using MF = Mfma<T>;
+	using acc4 = typename MF::acc;
+	constexpr int N = 1 << LOGN;
+	constexpr int LOGE = 2 * LOGN;
+	constexpr int E = 1 << LOGE;
+	constexpr int P = N + 1;
+	constexpr int EPT = E / kDenseThreads;
+	constexpr int NT = N / 16;
+	constexpr int TILES = NT * NT;
+	constexpr int MAXT = (TILES + 3) / 4;
+	__shared__ V2 s_rho[N * P];
+	__shared__ V2 s_h[N * P];
+	__shared__ V2 s_hlo[SPLIT ? N * P : 1];
+	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
+	__shared__ double2 s_selfx[SZ ? N : 1];
+	__shared__ int s_nb[2 * kTileMaxModes];
+
+	const int H = a.n_ado, M = a.M, B = a.B;
+	const int per_xcd = (H + kXcds - 1) / kXcds;
+	const int n = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+	if(n >= H) return;
+	const int tid = threadIdx.x;
+	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
+	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);
+	const V2* __restrict__ hd = static_cast<const V2*>(a.hdense);
+	const V2* __restrict__ hl = static_cast<const V2*>(a.hdense_lo);
+
+	if constexpr(SZ) {
+		for(int q = tid; q < N; q += kDenseThreads) s_selfx[q] = a.selfx[q];
+	} else {
+		for(int q = tid; q < B * N; q += kDenseThreads) s_v[q] = a.vdiag[q];
+	}
+	for(int q = tid; q < 2 * M; q += kDenseThreads) {
+		const int v = (q < M) ? a.up[n * M + q] : a.down[n * M + (q - M)];
+		s_nb[q] = (v < H) ? v : -1;
+	}
+	for(int e = tid; e < E; e += kDenseThreads) {
+		const int i = e >> LOGN, j = e & (N - 1);
+		s_rho[i * P + j] = own[e];
+		s_h[i * P + j] = hd[e];
+		if constexpr(SPLIT) s_hlo[i * P + j] = hl[e];
+	}
+	__syncthreads();
+
+	// G = H rho - rho H on MFMA
+	const int wave = tid >> 6, lane = tid & 63;
+	const int li = lane & 15, lk = lane >> 4;
+	acc4 gr[MAXT], gi[MAXT];
+#pragma unroll
+	for(int tt = 0; tt < MAXT; ++tt) {
+		const int t = wave + 4 * tt;
+		gr[tt] = acc4{};
+		gi[tt] = acc4{};
+		if(t >= TILES) continue;
+		const int I0 = (t / NT) * 16, J0 = (t % NT) * 16;
+		for(int k0 = 0; k0 < N; k0 += 4) {
+			const int kk = k0 + lk;
+			const V2 rb = s_rho[kk * P + J0 + li];
+			const V2 ra = s_rho[(I0 + li) * P + kk];
+#pragma unroll
+			for(int part = 0; part < (SPLIT ? 2 : 1); ++part) {
+				const V2* __restrict__ hs = part ? s_hlo : s_h;
+				const V2 ha = hs[(I0 + li) * P + kk];
+				const V2 hb = hs[kk * P + J0 + li];
+				// H rho
+				gr[tt] = MF::mma(ha.x, rb.x, gr[tt]);
+				gi[tt] = MF::mma(ha.x, rb.y, gi[tt]);
+				// - rho H
+				gr[tt] = MF::mma(-ra.x, hb.x, gr[tt]);
+				gi[tt] = MF::mma(-ra.y, hb.x, gi[tt]);
+				if(!a.hreal) {
+					gr[tt] = MF::mma(-ha.y, rb.y, gr[tt]);
+					gi[tt] = MF::mma(ha.y, rb.x, gi[tt]);
+					gr[tt] = MF::mma(ra.y, hb.y, gr[tt]);
+					gi[tt] = MF::mma(-ra.x, hb.y, gi[tt]);
+				}
+			}
+		}
+	}
+	__syncthreads();  // every MFMA operand read of rho is done
+
+	// rho_n <- -i G + (Sigma + Xi + counter) rho_n, at each wave's own tile positions
+	const double2 sf = a.selfc[n];
+#pragma unroll
+	for(int tt = 0; tt < MAXT; ++tt) {
+		const int t = wave + 4 * tt;
+		if(t >= TILES) continue;
+		const int I0 = (t / NT) * 16, J0 = (t % NT) * 16;
+#pragma unroll
+		for(int v = 0; v < 4; ++v) {
+			const int i = I0 + MF::row(lk, v), j = J0 + li;
+			double wr = sf.x, wi = sf.y;
+			if constexpr(SZ) {
+				const double2 x = s_selfx[i ^ j];
+				wr += x.x;
+				wi += x.y;
+			} else {
+				for(int b = 0; b < B; ++b) {
+					const double vi = s_v[b * N + i], vj = s_v[b * N + j];
+					const double d = vi - vj, s = vi + vj;
+					wr -= a.xi[b].x * d * d + a.xic[b].x * s * d;
+					wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
+				}
+			}
+			const V2 c = s_rho[i * P + j];
+			T vr = gi[tt][v], vi_ = -gr[tt][v];
+			cmadd<T, SPLIT>(vr, vi_, wr, wi, c.x, c.y);
+			s_rho[i * P + j] = V2{vr, vi_};
+		}
+	}
+	__syncthreads();
+
+	T ar[EPT], ai[EPT];
+#pragma unroll
+	for(int r = 0; r < EPT; ++r) {
+		const int e = tid + kDenseThreads * r;
+		const V2 c = s_rho[(e >> LOGN) * P + (e & (N - 1))];
+		ar[r] = c.x;
+		ai[r] = c.y;
+	}
+
+	for(int q = 0; q < 2 * M; ++q) {
+		const int idx = s_nb[q];
+		if(idx < 0) continue;
+		const bool down = q >= M;
+		const int k = down ? q - M : q;
+		const int b = a.mode_bath[k];
+		const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
+		V2 x[EPT];
+#pragma unroll
+		for(int r = 0; r < EPT; ++r) x[r] = src[tid + kDenseThreads * r];
+		if constexpr(SZ) {
+			const int mask = a.sz_mask[b];
+			const double two_alpha = 2.0 * a.sz_alpha[b];
+			double wcomm = -1.0, wanti = 0.0;
+			if(down) {
+				wcomm = a.dcomm[n * M + k].y;
+				wanti = a.danti[n * M + k].x;
+			}
+#pragma unroll
+			for(int r = 0; r < EPT; ++r) {
+				const int e = tid + kDenseThreads * r;
+				const int bi = (e >> LOGN) & mask, bj = e & mask;
+				const bool differ = bi != bj;
+				const double sg = bi ? two_alpha : -two_alpha;
+				const double w = sg * (differ ? wcomm : wanti);
+				const T u = differ ? -x[r].y : x[r].x;
+				const T v = differ ? x[r].x : x[r].y;
+				rmadd<T, SPLIT>(ar[r], ai[r], w, u, v);
+			}
+		} else {
+			double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
+			if(down) {
+				wc = a.dcomm[n * M + k];
+				wa = a.danti[n * M + k];
+			}
+			const double* __restrict__ vb = s_v + b * N;
+#pragma unroll
+			for(int r = 0; r < EPT; ++r) {
+				const int e = tid + kDenseThreads * r;
+				const double vi = vb[e >> LOGN], vj = vb[e & (N - 1)];
+				const double d = vi - vj, s = vi + vj;
+				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
+			}
+		}
+	}
+
+	const T sc = static_cast<T>(a.scale);
+	V2* __restrict__ out = static_cast<V2*>(a.out);
+#pragma unroll
+	for(int r = 0; r < EPT; ++r) {
+		const int e = tid + kDenseThreads * r;
+		const T tr = ar[r] * sc, ti = ai[r] * sc;
+		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
+		if constexpr(!LAST) {
+			out[g] = V2{tr, ti};
+		} else {
+			if(a.rho0_f64 != nullptr && n == 0) {
+				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
+				double2 rv = r0[e];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					rv.x += static_cast<double>(t.x);
+					rv.y += static_cast<double>(t.y);
+				}
+				rv.x += static_cast<double>(tr);
+				rv.y += static_cast<double>(ti);
+				r0[e] = rv;
+				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
+			} else {
+				V2 rv = static_cast<const V2*>(a.rho)[g];
+				for(int q = 0; q < a.ntprev; ++q) {
+					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					rv.x += t.x;
+					rv.y += t.y;
+				}
+				rv.x += tr;
+				rv.y += ti;
+				out[g] = rv;
+			}
+		}
+	}
+}
+
 // tier-0 block -> trace buffers (FP64): full block and/or real diagonal
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, double* rho_out,
@@ -524,6 +758,7 @@ struct helix_hip_plan {
 	double dt = 0.0;
 	bool flip = false;
 	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
+	bool dense = false;       // dense Hamiltonian on MFMA (N in {16, 32, 64})
 	size_t esize = 16;        // bytes per complex element
 	size_t elems = 0;         // H * N * N
 	void* rho = nullptr;
@@ -585,7 +820,15 @@ int validate(const helix_hip_problem* pr)
 	if(!(pr->dt > 0.0)) return fail(HELIX_HIP_EINVAL, "dt must be positive");
 	if(pr->precision < HELIX_HIP_FP64 || pr->precision > HELIX_HIP_SPLIT_FP32)
 		return fail(HELIX_HIP_EINVAL, "unknown precision");
-	if(pr->hamiltonian_dense) return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path not built in this library");
+	if(pr->hamiltonian_dense) {
+		if(pr->n != 16 && pr->n != 32 && pr->n != 64)
+			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports N = 16, 32, 64");
+		for(int b = 0; b < pr->n_baths; ++b)
+			if(pr->couplings[b].nflips != 0)
+				return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path needs diagonal couplings");
+		if(pr->n_baths > 16 || pr->n_modes > 16)
+			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports up to 16 baths and modes");
+	}
 	auto check_op = [&](const helix_hip_spin_operator& op, const char* name) -> int {
 		if(op.nflips < 0 || op.nflips > kMaxFlips) return fail(HELIX_HIP_EINVAL, std::string(name) + ": nflips > 16");
 		for(int f = 0; f < op.nflips; ++f)
@@ -652,10 +895,36 @@ void launch_tile(helix_hip_plan* p, const StageArgs& a, bool last)
 	}
 }
 
+template <typename T, bool SPLIT, int LOGN, bool SZ>
+void launch_dense_v(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	const unsigned grid = static_cast<unsigned>(((p->H + kXcds - 1) / kXcds) * kXcds);
+	if(last)
+		hipLaunchKernelGGL((k_stage_dense<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+	else
+		hipLaunchKernelGGL((k_stage_dense<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+}
+
+template <typename T, bool SPLIT, int LOGN>
+void launch_dense(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	if(a.sz)
+		launch_dense_v<T, SPLIT, LOGN, true>(p, a, last);
+	else
+		launch_dense_v<T, SPLIT, LOGN, false>(p, a, last);
+}
+
 template <typename T, bool SPLIT>
 void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 {
 	const unsigned grid = grid_for(p->elems);
+	if(p->dense) {
+		switch(p->logN) {
+		case 4: launch_dense<T, SPLIT, 4>(p, a, last); return;
+		case 5: launch_dense<T, SPLIT, 5>(p, a, last); return;
+		default: launch_dense<T, SPLIT, 6>(p, a, last); return;
+		}
+	}
 	if(p->tiled) {
 		switch(p->logN) {
 		case 3: launch_tile<T, SPLIT, 3>(p, a, last); return;
@@ -838,6 +1107,29 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		(rc = upload(p, vfm.data(), vfm.size(), &a.vflip_mask)) || (rc = upload(p, vfc.data(), vfc.size(), &a.vflip_coef)) ||
 		(rc = upload(p, pr->hamiltonian.diag, static_cast<size_t>(p->N), &a.hdiag)))
 		return bail(rc);
+	if(pr->hamiltonian_dense) {
+		const size_t n2 = static_cast<size_t>(p->N) * p->N;
+		p->dense = true;
+		a.hreal = 1;
+		for(size_t q = 0; q < n2; ++q)
+			if(pr->hamiltonian_dense[2 * q + 1] != 0.0) a.hreal = 0;
+		if(p->precision == HELIX_HIP_FP64) {
+			const double2* hd2 = nullptr;
+			if((rc = upload(p, reinterpret_cast<const double2*>(pr->hamiltonian_dense), n2, &hd2))) return bail(rc);
+			a.hdense = hd2;
+		} else {
+			std::vector<float> hi(2 * n2), lo(2 * n2);
+			for(size_t q = 0; q < 2 * n2; ++q) {
+				hi[q] = static_cast<float>(pr->hamiltonian_dense[q]);
+				lo[q] = static_cast<float>(pr->hamiltonian_dense[q] - static_cast<double>(hi[q]));
+			}
+			const float2* f2 = nullptr;
+			if((rc = upload(p, reinterpret_cast<const float2*>(hi.data()), n2, &f2))) return bail(rc);
+			a.hdense = f2;
+			if((rc = upload(p, reinterpret_cast<const float2*>(lo.data()), n2, &f2))) return bail(rc);
+			a.hdense_lo = f2;
+		}
+	}
 	a.hflips = pr->hamiltonian.nflips;
 	for(int f = 0; f < a.hflips; ++f) {
 		a.hmask[f] = pr->hamiltonian.flip_masks[f];
@@ -876,6 +1168,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		a.selfx = d2;
 	}
 	if(p->logN > 6 && !a.sz) p->tiled = false;  // see launch_tile
+	if(p->dense) p->tiled = false;
 	if(p->logN > 10) p->tiled = false;
 	a.buf = p->buf;
 	a.n_ado = p->H;
@@ -1030,7 +1323,7 @@ int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* laun
 	if(device_bytes) *device_bytes = p->device_bytes;
 	if(launches_per_step) *launches_per_step = p->order * (p->flip ? 2 : 1);
 	if(uses_flip_coupling) *uses_flip_coupling = p->flip ? 1 : 0;
-	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = 0;
+	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = p->dense ? 1 : 0;
 	return 0;
 }
 

@@ -408,6 +408,31 @@ Problem::Problem(SpinOperator hamiltonian, std::vector<BathSpec> baths, int tier
 	desc_.dt = dt;
 }
 
+void Problem::set_dense_hamiltonian(std::vector<cdouble> h)
+{
+	const size_t n2 = static_cast<size_t>(n_) * n_;
+	if(h.empty()) {
+		h.assign(n2, cdouble());
+		for(int i = 0; i < n_; ++i) {
+			h[static_cast<size_t>(i) * n_ + i] += ham_.diag[i];
+			for(size_t f = 0; f < ham_.masks.size(); ++f) h[static_cast<size_t>(i) * n_ + (i ^ ham_.masks[f])] += ham_.coefs[f];
+		}
+	}
+	if(h.size() != n2) throw std::invalid_argument("dense Hamiltonian must hold N*N entries");
+	hdense_.resize(2 * n2);
+	for(size_t q = 0; q < n2; ++q) {
+		hdense_[2 * q] = h[q].real();
+		hdense_[2 * q + 1] = h[q].imag();
+	}
+	desc_.hamiltonian_dense = hdense_.data();
+}
+
+void Problem::clear_dense_hamiltonian()
+{
+	hdense_.clear();
+	desc_.hamiltonian_dense = nullptr;
+}
+
 double Problem::energy_diagonal(const double* rho0) const
 {
 	double e = 0.0;
@@ -417,6 +442,16 @@ double Problem::energy_diagonal(const double* rho0) const
 
 double Problem::energy_trace(const double* rho0) const
 {
+	if(!hdense_.empty()) {
+		// Re Tr(H rho) = sum_ij Re(H_ij rho_ji)
+		double e = 0.0;
+		for(int i = 0; i < n_; ++i)
+			for(int j = 0; j < n_; ++j) {
+				const size_t hij = 2 * (static_cast<size_t>(i) * n_ + j), rji = 2 * (static_cast<size_t>(j) * n_ + i);
+				e += hdense_[hij] * rho0[rji] - hdense_[hij + 1] * rho0[rji + 1];
+			}
+		return e;
+	}
 	double e = energy_diagonal(rho0);
 	for(size_t f = 0; f < ham_.masks.size(); ++f)
 		for(int i = 0; i < n_; ++i) {

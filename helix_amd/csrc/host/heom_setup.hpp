@@ -103,6 +103,11 @@ public:
 	// <H> two ways: diagonal-only Re H_ii Re rho_ii (src/library/energy_observable.cpp:8-39) and Re Tr(H rho)
 	double energy_diagonal(const double* rho0) const;
 	double energy_trace(const double* rho0) const;
+	// Selects the dense (MFMA) commutator with H given as [N][N] complex, row-major.  An empty vector
+	// materialises the spin-structured Hamiltonian densely; clear_dense_hamiltonian() returns to it.
+	void set_dense_hamiltonian(std::vector<cdouble> h = {});
+	void clear_dense_hamiltonian();
+	bool dense_hamiltonian() const { return !hdense_.empty(); }
 
 private:
 	int n_ = 0;
@@ -112,6 +117,7 @@ private:
 	std::vector<int32_t> mode_bath_;
 	std::vector<double> down_comm_, down_anti_, self_, xi_, xic_;
 	std::vector<helix_hip_spin_operator> couplings_;
+	std::vector<double> hdense_;  // [N][N][2] when the dense path is selected
 	helix_hip_problem desc_{};
 };
 

@@ -78,6 +78,20 @@ int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam
 	}
 }
 
+int helix_model_use_dense_hamiltonian(helix_model* model, int enable)
+{
+	if(!model) return setup_fail(HELIX_HIP_EINVAL, "model is null");
+	try {
+		if(enable)
+			model->problem->set_dense_hamiltonian();
+		else
+			model->problem->clear_dense_hamiltonian();
+		return 0;
+	} catch(const std::exception& e) {
+		return setup_fail(HELIX_HIP_EINVAL, e.what());
+	}
+}
+
 const helix_hip_problem* helix_model_problem(const helix_model* model)
 {
 	return model ? &model->problem->descriptor() : nullptr;

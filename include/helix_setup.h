@@ -29,6 +29,10 @@ int helix_model_legacy(int n, double beta, double gamma, double zeta, int pade_t
 int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam, const double* gamma,
 	const double* temperature, int pade_terms, int max_tier, int order, double dt, int precision, helix_model** out);
 
+/* Selects (enable = 1) or deselects the dense MFMA commutator for H, materialising the model's
+ * Hamiltonian as [N][N] complex (N must be 16, 32 or 64 when the plan is created). */
+int helix_model_use_dense_hamiltonian(helix_model* model, int enable);
+
 const helix_hip_problem* helix_model_problem(const helix_model* model);
 int helix_model_rows(const helix_model* model, int32_t* rows /* [n_ado][n_modes] */);
 double helix_model_energy_diagonal(const helix_model* model, const double* rho0);

@@ -94,6 +94,35 @@ def test_precision_form_n256_full_size(hx):
         plan.close()
 
 
+@pytest.mark.parametrize("spins,L", [(4, 3), (5, 2), (6, 2)])
+def test_rhs_parity_dense_hamiltonian_mfma_fp64(hx, oracle_mod, spins, L):
+    """Dense H on MFMA (N = 16, 32, 64) against the oracle's CSR commutator."""
+    m = hx.Model.spin_glass(spins, seed=5, L=L, lam=0.2, gamma=0.9, T=0.8).use_dense_hamiltonian()
+    p = oracle_mod.spin_glass_problem(spins, 5, L=L, lam=0.2, gamma=0.9, T=0.8)
+    plan = hx.Plan(m)
+    assert plan.info()["dense"]
+    s = _rand_state(m.H, m.N, 31)
+    plan.set_state(s)
+    assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+
+
+def test_dense_hamiltonian_trajectories(hx, oracle_mod):
+    """4 spins, L=2 (45 ADOs of 16x16), 500 steps: FP64 dense path vs oracle, FP32/split dense vs oracle."""
+    p = oracle_mod.spin_glass_problem(4, 4, L=2)
+    ref = p.taylor(p.initial(), 0.01, 500)
+    for prec, tol in (("fp64", FP64_TOL), ("fp32", FORM_TOL), ("split_fp32", FORM_TOL)):
+        m = hx.Model.spin_glass(4, 4, L=2, dt=0.01, precision=prec).use_dense_hamiltonian()
+        plan = hx.Plan(m)
+        plan.set_initial()
+        plan.step(500)
+        if prec == "fp64":
+            assert _rel(plan.get_state(), ref) <= tol
+        else:
+            err = float(np.max(np.abs(plan.tier0() - ref[0])))
+            print(f"dense {prec}: max|d rho_S| = {err:.3e}")
+            assert err <= tol
+
+
 @pytest.mark.parametrize("N", [8, 32, 1024])
 def test_rhs_parity_legacy_fp64(hx, oracle_mod, N):
     m = hx.Model.legacy(N=N)
