@@ -22,10 +22,13 @@ PRECISIONS = {"fp64": FP64, "fp32": FP32, "split_fp32": SPLIT_FP32}
 HIP_SYMBOLS = [
     "helix_hip_plan_create", "helix_hip_set_initial", "helix_hip_set_state", "helix_hip_get_state", "helix_hip_rhs",
     "helix_hip_step", "helix_hip_get_tier0", "helix_hip_time_steps", "helix_hip_plan_info", "helix_hip_plan_destroy",
-    "helix_hip_last_error",
+    "helix_hip_last_error", "helix_hip_batch_create", "helix_hip_batch_set_initial", "helix_hip_batch_step",
+    "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
+    "helix_hip_batch_destroy",
 ]
 SETUP_SYMBOLS = [
-    "helix_model_legacy", "helix_model_spin_glass", "helix_model_use_dense_hamiltonian", "helix_model_problem",
+    "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_use_dense_hamiltonian",
+    "helix_model_problem",
     "helix_model_rows",
     "helix_model_energy_diagonal", "helix_model_energy_trace", "helix_model_free", "helix_setup_pade",
     "helix_setup_spin_glass_couplings", "helix_setup_hierarchy_size", "helix_setup_last_error",
@@ -83,6 +86,16 @@ def hip():
         L.helix_hip_plan_destroy.argtypes = [vp]
         L.helix_hip_plan_destroy.restype = None
         L.helix_hip_last_error.restype = C.c_char_p
+        L.helix_hip_batch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
+        L.helix_hip_batch_set_initial.argtypes = [vp, C.c_int, _dp]
+        L.helix_hip_batch_step.argtypes = [vp, C.c_uint64]
+        L.helix_hip_batch_get_tier0.argtypes = [vp, C.c_int, _dp]
+        L.helix_hip_batch_get_state.argtypes = [vp, C.c_int, _dp]
+        L.helix_hip_batch_time_steps.argtypes = [vp, C.c_uint64, _dp]
+        L.helix_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_size_t), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]
+        L.helix_hip_batch_destroy.argtypes = [vp]
+        L.helix_hip_batch_destroy.restype = None
         _hip = L
     return _hip
 
@@ -99,6 +112,8 @@ def host():
         L.helix_model_spin_glass.argtypes = [C.c_int, C.c_uint64, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int,
                                              C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
         L.helix_model_use_dense_hamiltonian.argtypes = [vp, C.c_int]
+        L.helix_model_spin_glass_random.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_int, C.POINTER(vp),
+                                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.helix_model_problem.argtypes = [vp]
         L.helix_model_problem.restype = C.POINTER(Problem)
         L.helix_model_rows.argtypes = [vp, _ip]
@@ -161,6 +176,17 @@ class Model:
                                            PRECISIONS[precision], C.byref(hdl))
         _check(rc, "helix_model_spin_glass", host())
         return cls(hdl, 1 << spins)
+
+    @classmethod
+    def spin_glass_random(cls, seed, order=4, dt=0.01, precision="fp64"):
+        """One instance of the batched workload (parameters drawn from SplitMix64(seed))."""
+        hdl = C.c_void_p()
+        sp, lt = C.c_int(), C.c_int()
+        _check(host().helix_model_spin_glass_random(seed, order, dt, PRECISIONS[precision], C.byref(hdl), C.byref(sp),
+                                                    C.byref(lt)), "helix_model_spin_glass_random", host())
+        m = cls(hdl, 1 << sp.value)
+        m.spins, m.max_tier = sp.value, lt.value
+        return m
 
     def use_dense_hamiltonian(self, enable: bool = True):
         """Route -i[H, rho] through the dense MFMA commutator (H materialised as N x N)."""
@@ -276,6 +302,62 @@ class Plan:
         if self._p:
             hip().helix_hip_plan_destroy(self._p)
             self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
+
+
+class Batch:
+    """Many independent problems advanced together: one launch per Taylor stage for all of them."""
+
+    def __init__(self, models, device: int = 0, graph: bool = True, autotune: bool = True):
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[C.cast(m.problem_ptr(), C.c_void_p) for m in self.models])
+        self._b = C.c_void_p()
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        _check(hip().helix_hip_batch_create(arr, len(self.models), device, flags, C.byref(self._b)),
+               "helix_hip_batch_create")
+
+    def set_initial(self, index: int = -1, rho0: np.ndarray | None = None):
+        r = None if rho0 is None else np.ascontiguousarray(rho0, np.complex128)
+        _check(hip().helix_hip_batch_set_initial(self._b, index, _pd(r.view(np.float64)) if r is not None else None),
+               "helix_hip_batch_set_initial")
+
+    def step(self, nsteps: int):
+        _check(hip().helix_hip_batch_step(self._b, nsteps), "helix_hip_batch_step")
+
+    def tier0(self, index: int) -> np.ndarray:
+        N = self.models[index].N
+        r = np.zeros((N, N), np.complex128)
+        _check(hip().helix_hip_batch_get_tier0(self._b, index, _pd(r.view(np.float64))), "batch_get_tier0")
+        return r
+
+    def state(self, index: int) -> np.ndarray:
+        m = self.models[index]
+        s = np.zeros((m.H, m.N, m.N), np.complex128)
+        _check(hip().helix_hip_batch_get_state(self._b, index, _pd(s.view(np.float64))), "batch_get_state")
+        return s
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = C.c_double()
+        _check(hip().helix_hip_batch_time_steps(self._b, nsteps, C.byref(ms)), "batch_time_steps")
+        return ms.value
+
+    def info(self) -> dict:
+        a, b, t, g = C.c_int64(), C.c_size_t(), C.c_int(), C.c_int()
+        _check(hip().helix_hip_batch_info(self._b, C.byref(a), C.byref(b), C.byref(t), C.byref(g)), "batch_info")
+        return dict(total_ado=a.value, device_bytes=b.value, threads=t.value, graph=bool(g.value))
+
+    def close(self):
+        if self._b:
+            hip().helix_hip_batch_destroy(self._b)
+            self._b = C.c_void_p()
 
     def __del__(self):
         try:

@@ -292,9 +292,18 @@ __device__ __forceinline__ void rmadd(T& ar, T& ai, double w, T xr, T xi)
 	}
 }
 
+// LDS a tile body needs (sized for the largest tile that uses it)
+template <typename T>
+struct TileLds {
+	using V2 = typename C2t<T>::type;
+	V2* blk;        // [kTile]
+	double* v;      // [16 N]   general diagonal couplings only
+	double2* selfx; // [N]      sigma^z couplings only
+	int* nb;        // [A][2 kTileMaxModes]
+};
+
 template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
-k_stage_tile(const StageArgs a, int ngroups)
+__device__ __forceinline__ void tile_body(const StageArgs& a, const int group, const TileLds<T>& lds)
 {
 	using V2 = typename C2t<T>::type;
 	constexpr int EPT = kTile / THREADS;
@@ -305,15 +314,10 @@ k_stage_tile(const StageArgs a, int ngroups)
 	constexpr int A = (E >= kTile) ? 1 : kTile / E;   // ADOs per workgroup (N < 64)
 	constexpr int TE = (E >= kTile) ? kTile : E;      // elements of one ADO held by the tile
 	constexpr int TA = THREADS / A;                   // threads per ADO
-	__shared__ V2 s_blk[kTile];
-	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
-	__shared__ double2 s_selfx[SZ ? N : 1];
-	__shared__ int s_nb[A][2 * kTileMaxModes];
-
-	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
-	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
-	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
-	if(group >= ngroups) return;
+	V2* __restrict__ s_blk = lds.blk;
+	double* __restrict__ s_v = lds.v;
+	double2* __restrict__ s_selfx = lds.selfx;
+	int (*s_nb)[2 * kTileMaxModes] = reinterpret_cast<int (*)[2 * kTileMaxModes]>(lds.nb);
 
 	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
@@ -484,6 +488,54 @@ k_stage_tile(const StageArgs a, int ngroups)
 				out[g] = rv;
 			}
 		}
+	}
+}
+
+template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
+k_stage_tile(const StageArgs a, int ngroups)
+{
+	using V2 = typename C2t<T>::type;
+	constexpr int N = 1 << LOGN;
+	constexpr int E = 1 << (2 * LOGN);
+	constexpr int A = (E >= kTile) ? 1 : kTile / E;
+	__shared__ V2 s_blk[kTile];
+	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
+	__shared__ double2 s_selfx[SZ ? N : 1];
+	__shared__ int s_nb[A * 2 * kTileMaxModes];
+	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
+	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
+	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+	if(group >= ngroups) return;
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb});
+}
+
+// Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
+// N <= 64 and sigma^z couplings; table[q] is instance q's argument block for this stage, tasks[t] =
+// (instance, tile group).  Tasks are numbered instance by instance, so the XCD renumbering keeps one
+// instance's tiles on one XCD.
+constexpr int kBatchMaxLogN = 6;
+
+template <typename T, bool SPLIT, bool LAST, int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
+k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ tasks, int ntasks)
+{
+	using V2 = typename C2t<T>::type;
+	__shared__ V2 s_blk[kTile];
+	__shared__ double s_v[1];
+	__shared__ double2 s_selfx[1 << kBatchMaxLogN];
+	__shared__ int s_nb[(kTile >> 6) * 2 * kTileMaxModes];
+	const int per_xcd = (ntasks + kXcds - 1) / kXcds;
+	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+	if(t >= ntasks) return;
+	const int2 tk = tasks[t];
+	const StageArgs& a = table[tk.x];
+	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb};
+	switch(a.logN) {
+	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true>(a, tk.y, lds); break;
+	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true>(a, tk.y, lds); break;
+	case 5: tile_body<T, SPLIT, LAST, 5, THREADS, true>(a, tk.y, lds); break;
+	default: tile_body<T, SPLIT, LAST, 6, THREADS, true>(a, tk.y, lds); break;
 	}
 }
 
@@ -1324,6 +1376,439 @@ int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* laun
 	if(launches_per_step) *launches_per_step = p->order * (p->flip ? 2 : 1);
 	if(uses_flip_coupling) *uses_flip_coupling = p->flip ? 1 : 0;
 	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = p->dense ? 1 : 0;
+	return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// Batched form
+// ------------------------------------------------------------------------------------------------
+
+namespace helix_amd {
+namespace {
+
+// sigma^z structure of a problem's couplings (see StageArgs::sz); fills masks/alpha and the
+// per-(i XOR j) self table.  Returns false when any coupling is not alpha (2 n_s - 1).
+bool sigma_z_structure(const helix_hip_problem* pr, int logN, StageArgs& a, std::vector<double>& selfx)
+{
+	const int N = pr->n;
+	if(pr->n_baths > 16) return false;
+	for(int b = 0; b < pr->n_baths; ++b) {
+		const helix_hip_spin_operator& op = pr->couplings[b];
+		if(op.nflips != 0) return false;
+		int found = -1;
+		for(int s = 0; s < logN && found < 0; ++s) {
+			const double alpha = op.diag[1 << s];
+			bool ok = alpha != 0.0;
+			for(int i = 0; i < N && ok; ++i) ok = op.diag[i] == (((i >> s) & 1) ? alpha : -alpha);
+			if(ok) found = s;
+		}
+		if(found < 0) return false;
+		a.sz_mask[b] = 1 << found;
+		a.sz_alpha[b] = op.diag[1 << found];
+	}
+	const size_t HM = static_cast<size_t>(pr->n_ado) * pr->n_modes;
+	for(size_t q = 0; q < HM; ++q)
+		if(pr->down_comm[2 * q] != 0.0 || pr->down_anti[2 * q + 1] != 0.0) return false;
+	selfx.assign(static_cast<size_t>(N) * 2, 0.0);
+	for(int x = 0; x < N; ++x)
+		for(int b = 0; b < pr->n_baths; ++b)
+			if(x & a.sz_mask[b]) {
+				const double d2 = 4.0 * a.sz_alpha[b] * a.sz_alpha[b];
+				selfx[2 * x] -= pr->xi[2 * b] * d2;
+				selfx[2 * x + 1] -= pr->xi[2 * b + 1] * d2;
+			}
+	a.sz = 1;
+	return true;
+}
+
+// host staging of many small tables uploaded as one device allocation
+struct Arena {
+	std::vector<char> bytes;
+	template <typename X>
+	size_t add(const X* p, size_t count)
+	{
+		const size_t off = (bytes.size() + 255) & ~size_t(255);
+		bytes.resize(off + count * sizeof(X));
+		if(count) std::memcpy(bytes.data() + off, p, count * sizeof(X));
+		return off;
+	}
+};
+
+}  // namespace
+}  // namespace helix_amd
+
+struct helix_hip_batch {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int Q = 0, order = 4, precision = 0;
+	size_t esize = 16;
+	std::vector<int> N, logN, H;
+	std::vector<size_t> elem_off, r0_off;
+	size_t total_elems = 0;
+	int64_t total_ado = 0;
+	void* rho = nullptr;
+	void* t[kMaxOrder + 1] = {};
+	double* rho0_f64 = nullptr;
+	StageArgs* table = nullptr;  // [order][Q]
+	int2* tasks = nullptr;
+	int ntasks = 0;
+	int threads = 0;
+	bool use_graph = false;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	std::vector<void*> allocations;
+	size_t device_bytes = 0;
+	std::vector<hipEvent_t> events;
+};
+
+namespace helix_amd {
+namespace {
+
+int batch_alloc(helix_hip_batch* b, void** ptr, size_t bytes)
+{
+	*ptr = nullptr;
+	if(bytes == 0) return 0;
+	hipError_t e = hipMalloc(ptr, bytes);
+	if(e != hipSuccess) return hip_fail(e, "hipMalloc");
+	b->allocations.push_back(*ptr);
+	b->device_bytes += bytes;
+	return 0;
+}
+
+template <typename T, bool SPLIT, int THREADS>
+void batch_launch_t(helix_hip_batch* b, int j)
+{
+	const unsigned grid = static_cast<unsigned>(((b->ntasks + kXcds - 1) / kXcds) * kXcds);
+	const StageArgs* tab = b->table + static_cast<size_t>(j - 1) * b->Q;
+	if(j == b->order)
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, true, THREADS>), dim3(grid), dim3(THREADS), 0, b->stream, tab, b->tasks,
+			b->ntasks);
+	else
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, false, THREADS>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
+			b->tasks, b->ntasks);
+}
+
+template <typename T, bool SPLIT>
+void batch_launch_p(helix_hip_batch* b, int j)
+{
+	if(b->threads == 1024)
+		batch_launch_t<T, SPLIT, 1024>(b, j);
+	else
+		batch_launch_t<T, SPLIT, 512>(b, j);
+}
+
+void batch_enqueue_step(helix_hip_batch* b)
+{
+	for(int j = 1; j <= b->order; ++j) {
+		switch(b->precision) {
+		case HELIX_HIP_FP64: batch_launch_p<double, false>(b, j); break;
+		case HELIX_HIP_FP32: batch_launch_p<float, false>(b, j); break;
+		default: batch_launch_p<float, true>(b, j); break;
+		}
+	}
+}
+
+int batch_time(helix_hip_batch* b, uint64_t nsteps, double* ms)
+{
+	hipEvent_t e0, e1;
+	HX_HIP(hipEventCreate(&e0));
+	HX_HIP(hipEventCreate(&e1));
+	HX_HIP(hipEventRecord(e0, b->stream));
+	for(uint64_t s = 0; s < nsteps; ++s) {
+		if(b->graph_exec)
+			HX_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+		else
+			batch_enqueue_step(b);
+	}
+	HX_HIP(hipEventRecord(e1, b->stream));
+	HX_HIP(hipEventSynchronize(e1));
+	float f = 0.f;
+	HX_HIP(hipEventElapsedTime(&f, e0, e1));
+	*ms = f;
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return hip_fail(e, "batch launch");
+	return 0;
+}
+
+}  // namespace
+}  // namespace helix_amd
+
+extern "C" {
+
+void helix_hip_batch_destroy(helix_hip_batch* b)
+{
+	if(!b) return;
+	if(b->stream) (void)hipStreamSynchronize(b->stream);
+	if(b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+	if(b->graph) (void)hipGraphDestroy(b->graph);
+	for(void* q : b->allocations) (void)hipFree(q);
+	if(b->stream) (void)hipStreamDestroy(b->stream);
+	delete b;
+}
+
+int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, int device, unsigned flags,
+	helix_hip_batch** out)
+{
+	if(!out || !problems || count <= 0) return fail(HELIX_HIP_EINVAL, "bad batch arguments");
+	*out = nullptr;
+	for(int q = 0; q < count; ++q) {
+		if(int rc = validate(problems[q])) return rc;
+		const helix_hip_problem* pr = problems[q];
+		if(pr->precision != problems[0]->precision || pr->order != problems[0]->order)
+			return fail(HELIX_HIP_EINVAL, "batched problems must share precision and order");
+		if(pr->order < 2) return fail(HELIX_HIP_EUNSUPPORTED, "batched form needs Taylor order >= 2");
+		if(pr->n > 64) return fail(HELIX_HIP_EUNSUPPORTED, "batched form supports N <= 64");
+		if(pr->hamiltonian_dense) return fail(HELIX_HIP_EUNSUPPORTED, "batched form uses spin-structured H");
+		if(pr->n_modes > kTileMaxModes) return fail(HELIX_HIP_EUNSUPPORTED, "batched form supports up to 16 modes");
+	}
+	HX_HIP(hipSetDevice(device));
+	helix_hip_batch* b = new(std::nothrow) helix_hip_batch();
+	if(!b) return fail(HELIX_HIP_ENOMEM, "host allocation failed");
+	auto bail = [&](int rc) {
+		helix_hip_batch_destroy(b);
+		return rc;
+	};
+	b->device = device;
+	b->Q = count;
+	b->order = problems[0]->order;
+	b->precision = problems[0]->precision;
+	b->esize = (b->precision == HELIX_HIP_FP64) ? 16 : 8;
+	if(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
+
+	// instance layout
+	size_t r0 = 0;
+	for(int q = 0; q < count; ++q) {
+		const helix_hip_problem* pr = problems[q];
+		const int lg = ilog2_exact(pr->n);
+		b->N.push_back(pr->n);
+		b->logN.push_back(lg);
+		b->H.push_back(pr->n_ado);
+		b->elem_off.push_back(b->total_elems);
+		b->total_elems += static_cast<size_t>(pr->n_ado) << (2 * lg);
+		b->total_ado += pr->n_ado;
+		b->r0_off.push_back(r0);
+		r0 += static_cast<size_t>(pr->n) * pr->n;
+	}
+	int rc = 0;
+	if((rc = batch_alloc(b, &b->rho, b->total_elems * b->esize))) return bail(rc);
+	for(int q = 1; q < b->order; ++q)
+		if((rc = batch_alloc(b, &b->t[q], b->total_elems * b->esize))) return bail(rc);
+	if(b->precision != HELIX_HIP_FP64) {
+		void* p0 = nullptr;
+		if((rc = batch_alloc(b, &p0, r0 * 16))) return bail(rc);
+		b->rho0_f64 = static_cast<double*>(p0);
+	}
+
+	// per-instance tables in one arena; argument blocks hold arena offsets until the upload
+	Arena arena;
+	struct Offs {
+		size_t up, down, mb, dcomm, danti, selfc, xi, xic, hdiag, selfx;
+	};
+	std::vector<StageArgs> base(count);
+	std::vector<Offs> offs(count);
+	std::vector<int2> tasks;
+	for(int q = 0; q < count; ++q) {
+		const helix_hip_problem* pr = problems[q];
+		StageArgs& a = base[q];
+		std::memset(&a, 0, sizeof(a));
+		std::vector<double> selfx;
+		if(!sigma_z_structure(pr, b->logN[q], a, selfx))
+			return bail(fail(HELIX_HIP_EUNSUPPORTED, "batched form needs sigma^z couplings"));
+		const size_t HM = static_cast<size_t>(pr->n_ado) * pr->n_modes;
+		Offs& o = offs[q];
+		o.up = arena.add(pr->up, HM);
+		o.down = arena.add(pr->down, HM);
+		o.mb = arena.add(pr->mode_bath, static_cast<size_t>(pr->n_modes));
+		o.dcomm = arena.add(pr->down_comm, 2 * HM);
+		o.danti = arena.add(pr->down_anti, 2 * HM);
+		o.selfc = arena.add(pr->self_coef, 2 * static_cast<size_t>(pr->n_ado));
+		o.xi = arena.add(pr->xi, 2 * static_cast<size_t>(pr->n_baths));
+		o.xic = arena.add(pr->xi_counter, 2 * static_cast<size_t>(pr->n_baths));
+		o.hdiag = arena.add(pr->hamiltonian.diag, static_cast<size_t>(pr->n));
+		o.selfx = arena.add(selfx.data(), selfx.size());
+		a.hflips = pr->hamiltonian.nflips;
+		for(int f = 0; f < a.hflips; ++f) {
+			a.hmask[f] = pr->hamiltonian.flip_masks[f];
+			a.hcoef[f] = pr->hamiltonian.flip_coefs[f];
+		}
+		a.n_ado = pr->n_ado;
+		a.M = pr->n_modes;
+		a.B = pr->n_baths;
+		a.logN = b->logN[q];
+		const int E = 1 << (2 * a.logN);
+		const int A = (E >= kTile) ? 1 : kTile / E;
+		const int ngroups = (pr->n_ado + A - 1) / A;
+		for(int g = 0; g < ngroups; ++g) tasks.push_back(make_int2(q, g));
+	}
+	void* dar = nullptr;
+	if((rc = batch_alloc(b, &dar, arena.bytes.size()))) return bail(rc);
+	HX_HIP(hipMemcpy(dar, arena.bytes.data(), arena.bytes.size(), hipMemcpyHostToDevice));
+	const char* ab = static_cast<const char*>(dar);
+	std::vector<StageArgs> table(static_cast<size_t>(b->order) * count);
+	for(int q = 0; q < count; ++q) {
+		StageArgs a = base[q];
+		const Offs& o = offs[q];
+		a.up = reinterpret_cast<const int32_t*>(ab + o.up);
+		a.down = reinterpret_cast<const int32_t*>(ab + o.down);
+		a.mode_bath = reinterpret_cast<const int32_t*>(ab + o.mb);
+		a.dcomm = reinterpret_cast<const double2*>(ab + o.dcomm);
+		a.danti = reinterpret_cast<const double2*>(ab + o.danti);
+		a.selfc = reinterpret_cast<const double2*>(ab + o.selfc);
+		a.xi = reinterpret_cast<const double2*>(ab + o.xi);
+		a.xic = reinterpret_cast<const double2*>(ab + o.xic);
+		a.hdiag = reinterpret_cast<const double*>(ab + o.hdiag);
+		a.selfx = reinterpret_cast<const double2*>(ab + o.selfx);
+		const size_t off = b->elem_off[q] * b->esize;
+		auto at = [&](void* base_ptr) { return static_cast<void*>(static_cast<char*>(base_ptr) + off); };
+		for(int j = 1; j <= b->order; ++j) {
+			StageArgs s = a;
+			s.scale = problems[q]->dt / j;
+			s.cur = (j == 1) ? at(b->rho) : at(b->t[j - 1]);
+			if(j < b->order) {
+				s.out = at(b->t[j]);
+			} else {
+				s.out = at(b->rho);
+				s.rho = at(b->rho);
+				s.ntprev = b->order - 1;
+				for(int k = 0; k < s.ntprev; ++k) s.tprev[k] = at(b->t[k + 1]);
+				s.rho0_f64 = b->rho0_f64 ? b->rho0_f64 + 2 * b->r0_off[q] : nullptr;
+			}
+			table[static_cast<size_t>(j - 1) * count + q] = s;
+		}
+	}
+	void* dt = nullptr;
+	if((rc = batch_alloc(b, &dt, table.size() * sizeof(StageArgs)))) return bail(rc);
+	HX_HIP(hipMemcpy(dt, table.data(), table.size() * sizeof(StageArgs), hipMemcpyHostToDevice));
+	b->table = static_cast<StageArgs*>(dt);
+	void* dk = nullptr;
+	if((rc = batch_alloc(b, &dk, tasks.size() * sizeof(int2)))) return bail(rc);
+	HX_HIP(hipMemcpy(dk, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice));
+	b->tasks = static_cast<int2*>(dk);
+	b->ntasks = static_cast<int>(tasks.size());
+	HX_HIP(hipMemsetAsync(b->rho, 0, b->total_elems * b->esize, b->stream));
+	for(int q = 1; q < b->order; ++q) HX_HIP(hipMemsetAsync(b->t[q], 0, b->total_elems * b->esize, b->stream));
+	if(b->rho0_f64) HX_HIP(hipMemsetAsync(b->rho0_f64, 0, r0 * 16, b->stream));
+	HX_HIP(hipStreamSynchronize(b->stream));
+
+	// workgroup shape: complex128 tiles default to 512 threads, complex64 to 1024; the autotuner times
+	// both on the all-zero state (stepping zero leaves it zero) and keeps the faster
+	b->threads = (b->precision == HELIX_HIP_FP64) ? 512 : 1024;
+	if(flags & HELIX_HIP_BATCH_AUTOTUNE) {
+		double best = 1e300;
+		int pick = b->threads;
+		for(int th : {512, 1024}) {
+			b->threads = th;
+			double ms = 0.0;
+			if((rc = batch_time(b, 1, &ms)) || (rc = batch_time(b, 3, &ms))) return bail(rc);
+			if(ms < best) {
+				best = ms;
+				pick = th;
+			}
+		}
+		b->threads = pick;
+	}
+	if(flags & HELIX_HIP_BATCH_GRAPH) {
+		HX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+		batch_enqueue_step(b);
+		hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+		if(e != hipSuccess) return bail(hip_fail(e, "hipStreamEndCapture"));
+		e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0);
+		if(e != hipSuccess) return bail(hip_fail(e, "hipGraphInstantiate"));
+		b->use_graph = true;
+	}
+	*out = b;
+	return 0;
+}
+
+int helix_hip_batch_set_initial(helix_hip_batch* b, int index, const double* rho0)
+{
+	if(!b || index >= b->Q || (index >= 0 && !rho0)) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	HX_HIP(hipStreamSynchronize(b->stream));
+	const int q0 = index < 0 ? 0 : index, q1 = index < 0 ? b->Q : index + 1;
+	for(int q = q0; q < q1; ++q) {
+		const size_t n2 = static_cast<size_t>(b->N[q]) * b->N[q];
+		const size_t elems = static_cast<size_t>(b->H[q]) * n2;
+		std::vector<double> r(n2 * 2, 0.0);
+		if(index < 0)
+			r[2 * (n2 - 1)] = 1.0;  // |N-1><N-1|
+		else
+			std::memcpy(r.data(), rho0, n2 * 16);
+		char* base = static_cast<char*>(b->rho) + b->elem_off[q] * b->esize;
+		HX_HIP(hipMemset(base, 0, elems * b->esize));
+		if(b->precision == HELIX_HIP_FP64) {
+			HX_HIP(hipMemcpy(base, r.data(), n2 * 16, hipMemcpyHostToDevice));
+		} else {
+			std::vector<float> f(n2 * 2);
+			for(size_t k = 0; k < n2 * 2; ++k) f[k] = static_cast<float>(r[k]);
+			HX_HIP(hipMemcpy(base, f.data(), n2 * 8, hipMemcpyHostToDevice));
+			HX_HIP(hipMemcpy(b->rho0_f64 + 2 * b->r0_off[q], r.data(), n2 * 16, hipMemcpyHostToDevice));
+		}
+	}
+	return 0;
+}
+
+int helix_hip_batch_step(helix_hip_batch* b, uint64_t nsteps)
+{
+	if(!b) return fail(HELIX_HIP_EINVAL, "batch is null");
+	for(uint64_t s = 0; s < nsteps; ++s) {
+		if(b->graph_exec)
+			HX_HIP(hipGraphLaunch(b->graph_exec, b->stream));
+		else
+			batch_enqueue_step(b);
+	}
+	if(int rc = launch_error("batch step")) return rc;
+	HX_HIP(hipStreamSynchronize(b->stream));
+	return 0;
+}
+
+int helix_hip_batch_get_tier0(helix_hip_batch* b, int index, double* out)
+{
+	if(!b || !out || index < 0 || index >= b->Q) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	HX_HIP(hipStreamSynchronize(b->stream));
+	const size_t n2 = static_cast<size_t>(b->N[index]) * b->N[index];
+	if(b->precision == HELIX_HIP_FP64)
+		HX_HIP(hipMemcpy(out, static_cast<char*>(b->rho) + b->elem_off[index] * 16, n2 * 16, hipMemcpyDeviceToHost));
+	else
+		HX_HIP(hipMemcpy(out, b->rho0_f64 + 2 * b->r0_off[index], n2 * 16, hipMemcpyDeviceToHost));
+	return 0;
+}
+
+int helix_hip_batch_get_state(helix_hip_batch* b, int index, double* state)
+{
+	if(!b || !state || index < 0 || index >= b->Q) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	HX_HIP(hipStreamSynchronize(b->stream));
+	const size_t n2 = static_cast<size_t>(b->N[index]) * b->N[index];
+	const size_t elems = static_cast<size_t>(b->H[index]) * n2;
+	const char* base = static_cast<const char*>(b->rho) + b->elem_off[index] * b->esize;
+	if(b->precision == HELIX_HIP_FP64) {
+		HX_HIP(hipMemcpy(state, base, elems * 16, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> f(elems * 2);
+		HX_HIP(hipMemcpy(f.data(), base, elems * 8, hipMemcpyDeviceToHost));
+		for(size_t k = 0; k < elems * 2; ++k) state[k] = static_cast<double>(f[k]);
+		HX_HIP(hipMemcpy(state, b->rho0_f64 + 2 * b->r0_off[index], n2 * 16, hipMemcpyDeviceToHost));
+	}
+	return 0;
+}
+
+int helix_hip_batch_time_steps(helix_hip_batch* b, uint64_t nsteps, double* total_ms)
+{
+	if(!b || !total_ms) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	return batch_time(b, nsteps, total_ms);
+}
+
+int helix_hip_batch_info(const helix_hip_batch* b, int64_t* total_ado, size_t* device_bytes, int* threads, int* graph)
+{
+	if(!b) return fail(HELIX_HIP_EINVAL, "batch is null");
+	if(total_ado) *total_ado = b->total_ado;
+	if(device_bytes) *device_bytes = b->device_bytes;
+	if(threads) *threads = b->threads;
+	if(graph) *graph = b->use_graph ? 1 : 0;
 	return 0;
 }
 

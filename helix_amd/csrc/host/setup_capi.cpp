@@ -2,6 +2,7 @@
 #include "heom_setup.hpp"
 #include "helix_setup.h"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -76,6 +77,21 @@ int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam
 	} catch(const std::exception& e) {
 		return setup_fail(HELIX_HIP_EINVAL, e.what());
 	}
+}
+
+int helix_model_spin_glass_random(uint64_t seed, int order, double dt, int precision, helix_model** out,
+	int* spins_out, int* max_tier_out)
+{
+	SplitMix64 rng(seed);
+	const int spins = 3 + std::min(static_cast<int>(rng.uniform() * 4.0), 3);
+	const int max_tier = 2 + std::min(static_cast<int>(rng.uniform() * 4.0), 3);
+	const double lam = 0.05 + 0.45 * rng.uniform();
+	const double gamma = 0.5 + 1.5 * rng.uniform();
+	const double temp = 0.5 + 1.5 * rng.uniform();
+	if(spins_out) *spins_out = spins;
+	if(max_tier_out) *max_tier_out = max_tier;
+	const std::vector<double> l(spins, lam), g(spins, gamma), t(spins, temp);
+	return helix_model_spin_glass(spins, seed, 1.0, l.data(), g.data(), t.data(), 1, max_tier, order, dt, precision, out);
 }
 
 int helix_model_use_dense_hamiltonian(helix_model* model, int enable)

@@ -119,6 +119,28 @@ int helix_hip_plan_info(const helix_hip_plan* plan, size_t* device_bytes, int* l
 
 void helix_hip_plan_destroy(helix_hip_plan* plan);
 
+/* ---- batched form: many independent problems advanced by one launch per Taylor stage ----
+ * No counterpart in the reference (one hard-coded problem per process, src/helix.cpp:73-104).
+ * Every problem must have N <= 64, sigma^z couplings, a spin-structured H, and the same precision
+ * and Taylor order (>= 2); dt and everything else may differ.  Flags: */
+#define HELIX_HIP_BATCH_GRAPH 1u     /* capture the m stage launches of a step in a hipGraph */
+#define HELIX_HIP_BATCH_AUTOTUNE 2u  /* time the workgroup shapes at creation and keep the fastest */
+
+typedef struct helix_hip_batch helix_hip_batch;
+
+int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, int device, unsigned flags,
+	helix_hip_batch** out);
+/* index < 0: every problem gets rho_0 = |N-1><N-1| (rho0 ignored); else problem `index` gets rho0. */
+int helix_hip_batch_set_initial(helix_hip_batch* batch, int index, const double* rho0);
+int helix_hip_batch_step(helix_hip_batch* batch, uint64_t nsteps);
+int helix_hip_batch_get_tier0(helix_hip_batch* batch, int index, double* out);
+int helix_hip_batch_get_state(helix_hip_batch* batch, int index, double* state);
+int helix_hip_batch_time_steps(helix_hip_batch* batch, uint64_t nsteps, double* total_ms);
+/* total ADOs, device bytes, workgroup threads chosen, graph in use */
+int helix_hip_batch_info(const helix_hip_batch* batch, int64_t* total_ado, size_t* device_bytes, int* threads,
+	int* graph);
+void helix_hip_batch_destroy(helix_hip_batch* batch);
+
 /* Message of the calling thread's last error ("" when none). */
 const char* helix_hip_last_error(void);
 

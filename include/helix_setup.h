@@ -29,6 +29,13 @@ int helix_model_legacy(int n, double beta, double gamma, double zeta, int pade_t
 int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam, const double* gamma,
 	const double* temperature, int pade_terms, int max_tier, int order, double dt, int precision, helix_model** out);
 
+/* One instance of the batched workload: from SplitMix64(seed) draw, in this order, spins = 3 + floor(4u),
+ * max_tier = 2 + floor(4u), lambda = 0.05 + 0.45u, gamma = 0.5 + 1.5u, T = 0.5 + 1.5u (every spin's bath
+ * gets the same lambda, gamma, T); J_ij from a fresh SplitMix64(seed) stream as in helix_model_spin_glass,
+ * h = 1, Pade K = 1.  spins_out / max_tier_out may be NULL. */
+int helix_model_spin_glass_random(uint64_t seed, int order, double dt, int precision, helix_model** out,
+	int* spins_out, int* max_tier_out);
+
 /* Selects (enable = 1) or deselects the dense MFMA commutator for H, materialising the model's
  * Hamiltonian as [N][N] complex (N must be 16, 32 or 64 when the plan is created). */
 int helix_model_use_dense_hamiltonian(helix_model* model, int enable);
