@@ -20,7 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["heom_plan.hip"]
+HIP_SOURCES = ["heom_plan.hip", "mlp.hip"]
 HOST_SOURCES = ["host/heom_setup.cpp", "host/setup_capi.cpp", "host/helix_api.cpp"]
 
 HIPCC_FLAGS = [

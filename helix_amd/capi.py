@@ -24,7 +24,7 @@ HIP_SYMBOLS = [
     "helix_hip_step", "helix_hip_get_tier0", "helix_hip_time_steps", "helix_hip_plan_info", "helix_hip_plan_destroy",
     "helix_hip_last_error", "helix_hip_batch_create", "helix_hip_batch_set_initial", "helix_hip_batch_step",
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
-    "helix_hip_batch_destroy",
+    "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_use_dense_hamiltonian",

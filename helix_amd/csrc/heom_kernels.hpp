@@ -60,6 +60,9 @@ struct StageArgs {
 	const void* hdense;
 	const void* hdense_lo;
 	int hreal;                // every Im H_ij == 0: the imaginary-H products are skipped
+	// learned closure of the deepest kept tier: L(rho)_n += kappa[kslot[n]][e] rho_n[e] where kslot[n] >= 0
+	const double2* kappa;     // [count][N^2] or null
+	const int32_t* kslot;     // [H]
 };
 
 }  // namespace helix_amd

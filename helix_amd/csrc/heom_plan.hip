@@ -6,6 +6,7 @@
 #include "heom_kernels.hpp"
 
 #include "helix_hip.h"
+#include "helix_mlp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -201,9 +202,17 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 			}
 		}
 
-		// Sigma, Xi and counter terms
+		// Sigma, Xi and counter terms (+ the learned closure rate of the deepest kept tier)
 		{
-			const double2 sc = a.selfc[n];
+			double2 sc = a.selfc[n];
+			if(a.kappa) {
+				const int sl = a.kslot[n];
+				if(sl >= 0) {
+					const double2 kk = a.kappa[(static_cast<size_t>(sl) << logE) + e];
+					sc.x += kk.x;
+					sc.y += kk.y;
+				}
+			}
 			if constexpr(!FLIP) {
 				double wr = sc.x, wi = sc.y;
 				for(int b = 0; b < a.B; ++b) {
@@ -448,6 +457,14 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
 			}
 		}
+		if(a.kappa) {
+			const int sl = a.kslot[n];
+			if(sl >= 0) {
+				const double2 kk = a.kappa[(static_cast<size_t>(sl) << LOGE) + e];
+				wr += kk.x;
+				wi += kk.y;
+			}
+		}
 		const V2 c = blk[le];
 		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c.x, c.y);
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
@@ -673,6 +690,14 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 					wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
 				}
 			}
+			if(a.kappa) {
+				const int sl = a.kslot[n];
+				if(sl >= 0) {
+					const double2 kk = a.kappa[(static_cast<size_t>(sl) << LOGE) + i * N + j];
+					wr += kk.x;
+					wi += kk.y;
+				}
+			}
 			const V2 c = s_rho[i * P + j];
 			T vr = gi[tt][v], vi_ = -gr[tt][v];
 			cmadd<T, SPLIT>(vr, vi_, wr, wi, c.x, c.y);
@@ -773,6 +798,34 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	}
 }
 
+// Learned closure: feature rows for every (listed ADO a, element e = (i, j)):
+//   x = [n_b(i) for the logN spins (as +-1), n_b(j) likewise, extra[a][0 .. extra_dim)]
+__global__ __launch_bounds__(kBlock) void k_kappa_features(int count, int logN, const float* __restrict__ extra,
+	int extra_dim, float* __restrict__ x)
+{
+	const int E = 1 << (2 * logN);
+	const int din = 2 * logN + extra_dim;
+	const size_t rows = static_cast<size_t>(count) * E;
+	for(size_t r = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; r < rows; r += gridDim.x * static_cast<size_t>(kBlock)) {
+		const int a = static_cast<int>(r >> (2 * logN));
+		const int e = static_cast<int>(r & (E - 1));
+		const int i = e >> logN, j = e & ((1 << logN) - 1);
+		float* row = x + r * din;
+		for(int b = 0; b < logN; ++b) {
+			row[b] = ((i >> b) & 1) ? 1.f : -1.f;
+			row[logN + b] = ((j >> b) & 1) ? 1.f : -1.f;
+		}
+		for(int q = 0; q < extra_dim; ++q) row[2 * logN + q] = extra[static_cast<size_t>(a) * extra_dim + q];
+	}
+}
+
+// kappa[a][e] = y[a E + e][0] + i y[a E + e][1]
+__global__ __launch_bounds__(kBlock) void k_kappa_store(size_t rows, const float* __restrict__ y, double2* __restrict__ kappa)
+{
+	for(size_t r = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; r < rows; r += gridDim.x * static_cast<size_t>(kBlock))
+		kappa[r] = make_double2(static_cast<double>(y[2 * r]), static_cast<double>(y[2 * r + 1]));
+}
+
 // tier-0 block -> trace buffers (FP64): full block and/or real diagonal
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, double* rho_out,
@@ -821,7 +874,17 @@ struct helix_hip_plan {
 	size_t device_bytes = 0;
 	StageArgs base{};
 	std::vector<hipEvent_t> events;
+	// learned closure of the deepest kept tier (helix_hip_plan_set_closure)
+	double2* kappa = nullptr;
+	int32_t* kslot = nullptr;
+	int kappa_count = 0;
 };
+
+namespace helix_amd_mlp {
+float* input_rows(helix_hip_mlp* m, int n);
+const float* forward_on(helix_hip_mlp* m, int n, hipStream_t stream);
+int dims(const helix_hip_mlp* m, int* in, int* out);
+}  // namespace helix_amd_mlp
 
 namespace {
 
@@ -1365,6 +1428,87 @@ int helix_hip_time_steps(helix_hip_plan* p, uint64_t nsteps, double* stage_ms, d
 		HX_HIP(hipEventElapsedTime(&ms, p->events[0], p->events[q - 1]));
 		*total_ms = ms;
 	}
+	return 0;
+}
+
+int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int32_t* ados, int count, const float* extra,
+	int extra_dim)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	if(!mlp) {
+		p->base.kappa = nullptr;
+		p->base.kslot = nullptr;
+		return 0;
+	}
+	if(!ados || count <= 0 || extra_dim < 0 || (extra_dim > 0 && !extra)) return fail(HELIX_HIP_EINVAL, "bad closure");
+	int din = 0, dout = 0;
+	helix_amd_mlp::dims(mlp, &din, &dout);
+	if(din != 2 * p->logN + extra_dim || dout != 2) return fail(HELIX_HIP_EINVAL, "MLP sizes must be (2 log2 N + extra, 2)");
+	std::vector<int32_t> slot(p->H, -1);
+	for(int a = 0; a < count; ++a) {
+		if(ados[a] < 0 || ados[a] >= p->H) return fail(HELIX_HIP_EINVAL, "closure ADO index out of range");
+		slot[ados[a]] = a;
+	}
+	const size_t E = static_cast<size_t>(p->N) * p->N;
+	const size_t rows = static_cast<size_t>(count) * E;
+	int rc = 0;
+	void* q = nullptr;
+	if(!p->kslot || p->kappa_count < count) {
+		if((rc = plan_alloc(p, &q, sizeof(int32_t) * p->H))) return rc;
+		p->kslot = static_cast<int32_t*>(q);
+		if((rc = plan_alloc(p, &q, sizeof(double2) * rows))) return rc;
+		p->kappa = static_cast<double2*>(q);
+		p->kappa_count = count;
+	}
+	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * p->H, hipMemcpyHostToDevice));
+	float* dextra = nullptr;
+	if(extra_dim > 0) {
+		HX_HIP(hipMalloc(&dextra, sizeof(float) * count * extra_dim));
+		HX_HIP(hipMemcpy(dextra, extra, sizeof(float) * count * extra_dim, hipMemcpyHostToDevice));
+	}
+	float* x = helix_amd_mlp::input_rows(mlp, static_cast<int>(rows));
+	if(!x) return fail(HELIX_HIP_ENOMEM, "closure workspace");
+	hipLaunchKernelGGL(k_kappa_features, dim3(grid_for(rows)), dim3(kBlock), 0, p->stream, count, p->logN, dextra, extra_dim, x);
+	const float* y = helix_amd_mlp::forward_on(mlp, static_cast<int>(rows), p->stream);
+	hipLaunchKernelGGL(k_kappa_store, dim3(grid_for(rows)), dim3(kBlock), 0, p->stream, rows, y, p->kappa);
+	if(int e = launch_error("closure launch")) return e;
+	HX_HIP(hipStreamSynchronize(p->stream));
+	if(dextra) (void)hipFree(dextra);
+	p->base.kappa = p->kappa;
+	p->base.kslot = p->kslot;
+	return 0;
+}
+
+int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int count, const double* kappa)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	if(!kappa) {
+		p->base.kappa = nullptr;
+		p->base.kslot = nullptr;
+		return 0;
+	}
+	if(!ados || count <= 0) return fail(HELIX_HIP_EINVAL, "bad closure rates");
+	std::vector<int32_t> slot(p->H, -1);
+	for(int a = 0; a < count; ++a) {
+		if(ados[a] < 0 || ados[a] >= p->H) return fail(HELIX_HIP_EINVAL, "closure ADO index out of range");
+		slot[ados[a]] = a;
+	}
+	const size_t rows = static_cast<size_t>(count) * p->N * p->N;
+	int rc = 0;
+	void* q = nullptr;
+	if(!p->kslot || p->kappa_count < count) {
+		if((rc = plan_alloc(p, &q, sizeof(int32_t) * p->H))) return rc;
+		p->kslot = static_cast<int32_t*>(q);
+		if((rc = plan_alloc(p, &q, sizeof(double2) * rows))) return rc;
+		p->kappa = static_cast<double2*>(q);
+		p->kappa_count = count;
+	}
+	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * p->H, hipMemcpyHostToDevice));
+	HX_HIP(hipMemcpy(p->kappa, kappa, sizeof(double2) * rows, hipMemcpyHostToDevice));
+	p->base.kappa = p->kappa;
+	p->base.kslot = p->kslot;
 	return 0;
 }
 

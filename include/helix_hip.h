@@ -119,6 +119,21 @@ int helix_hip_plan_info(const helix_hip_plan* plan, size_t* device_bytes, int* l
 
 void helix_hip_plan_destroy(helix_hip_plan* plan);
 
+/* Learned closure of the deepest kept tier (the network of include/helix_mlp.h).  No counterpart in the
+ * reference, whose deepest tier simply loses its up-neighbour terms (src/liouville.cu:818-823).  The
+ * missing term of each listed ADO n = ados[a] is modelled as an elementwise complex rate,
+ *   L(rho)_n[ij] += kappa_a[ij] rho_n[ij],   (Re, Im) kappa_a[ij] = f(x),
+ *   x = [2 n_b(i) - 1 for b < log2 N, 2 n_b(j) - 1 for b < log2 N, extra[a][0 .. extra_dim)]
+ * (the form of a Markovian terminator for sigma^z couplings).  The network runs once here, on the GPU,
+ * and kappa stays resident; every stage kernel then applies it (linear, so exact in every Taylor stage).
+ * mlp == NULL switches the closure off (plain truncation).  The MLP may be destroyed afterwards. */
+struct helix_hip_mlp;
+/* Same closure with the rates given directly: kappa [count][N][N][2] (e.g. an analytic terminator);
+ * kappa == NULL switches it off. */
+int helix_hip_plan_set_closure_rates(helix_hip_plan* plan, const int32_t* ados, int count, const double* kappa);
+int helix_hip_plan_set_closure(helix_hip_plan* plan, struct helix_hip_mlp* mlp, const int32_t* ados, int count,
+	const float* extra, int extra_dim);
+
 /* ---- batched form: many independent problems advanced by one launch per Taylor stage ----
  * No counterpart in the reference (one hard-coded problem per process, src/helix.cpp:73-104).
  * Every problem must have N <= 64, sigma^z couplings, a spin-structured H, and the same precision

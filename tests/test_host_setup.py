@@ -35,8 +35,11 @@ def test_libraries_export_every_declared_symbol(hx):
         assert hasattr(hip, name), name
     for name in _declared("helix_setup.h"):
         assert hasattr(host, name), name
+    for name in _declared("helix_mlp.h"):
+        assert hasattr(hip, name), name
     assert set(_declared("helix_hip.h")) == set(capi.HIP_SYMBOLS)
     assert set(_declared("helix_setup.h")) == set(capi.SETUP_SYMBOLS)
+    assert len(_declared("helix_mlp.h")) == 9
 
 
 def test_pade_reference_values(hx):
