@@ -56,6 +56,63 @@ def cpu_baseline(spec, steps: int = 1):
             "sample": f"{steps} Taylor-4 step(s) of the full workload ({p.H} ADOs, N={p.N}) in {sec:.2f} s"}
 
 
+def _profiled_traffic(workload, precision):
+    """Mean HBM bytes per stage launch from the committed rocprofv3 counter summary, when it covers this
+    workload and precision."""
+    path = os.path.join(ROOT, "profiles", "stage_traffic_spin_glass_n64_L5.json")
+    if workload != "spin_glass_n64_L5" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    t = d.get("traffic", {}).get(precision)
+    return None if t is None else t["mean_bytes_per_launch"]
+
+
+def _time_single(model_kwargs, steps, warmup=10):
+    import helix_amd as hx
+    m = hx.Model.spin_glass(**model_kwargs)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(warmup)
+    t0 = time.perf_counter()
+    plan.step(steps)
+    sec = time.perf_counter() - t0
+    plan.close()
+    return m.H * steps / sec
+
+
+def _time_batch(precision, count, steps, warmup=2):
+    import helix_amd as hx
+    models = [hx.Model.spin_glass_random(1000 + i, dt=0.01, precision=precision) for i in range(count)]
+    b = hx.Batch(models, graph=True, autotune=True)
+    b.set_initial()
+    b.step(warmup)
+    t0 = time.perf_counter()
+    b.step(steps)
+    sec = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    return info["total_ado"] * steps / sec, info
+
+
+def measure_forms(args, fp64_value):
+    """The chosen forms beside the plain FP64 line: complex64 and split-FP32 on the same workload, and the
+    512-instance batched workload in FP64 and complex64 (ADO-steps/s and the ratio to plain FP64 of the
+    same workload)."""
+    spins, seed, L, dt, lam, gamma, T, K = WORKLOADS[args.workload]
+    kw = dict(spins=spins, seed=seed, lam=lam, gamma=gamma, T=T, K=K, L=L, dt=dt)
+    forms = {}
+    for prec in ("fp32", "split_fp32"):
+        v = _time_single(dict(kw, precision=prec), args.form_steps)
+        forms[prec] = {"value": v, "x_plain_fp64": v / fp64_value}
+    v64, info = _time_batch("fp64", 512, args.batch_steps)
+    v32, _ = _time_batch("fp32", 512, args.batch_steps)
+    forms["batched_512"] = {"n_ado": info["total_ado"], "fp64": v64, "complex64": v32,
+                            "complex64_x_batched_fp64": v32 / v64, "complex64_x_plain_fp64_n64": v32 / fp64_value}
+    print(f"# forms: {json.dumps(forms)}", file=sys.stderr, flush=True)
+    return forms
+
+
 def copy_probe(nbytes: int) -> int:
     """One hipMemcpy device-to-device of nbytes: a plain copy kernel of known size, so rocprof's
     FETCH_SIZE / WRITE_SIZE for it give the counter scale (bytes read = bytes written = nbytes)."""
@@ -71,6 +128,86 @@ def copy_probe(nbytes: int) -> int:
     hip.hipFree(a)
     hip.hipFree(b)
     return nbytes
+
+
+def bench_learned(args):
+    """Learned-closure form: 4 spins (N = 16), tiers 0..2 kept plus the learned closure, complex64,
+    dt 0.01, 2000 steps, against plain FP64 at tiers 0..6 (3003 ADOs) on held-out parameter sets.
+    Training (128 parameter sets propagated by the deep plain solver, 20 epochs on the GPU) is done first
+    and not timed.  value = ADO-steps/s of the learned solver; time_to_solution_x = deep plain time /
+    learned time for the same 2000 steps."""
+    import numpy as np
+
+    import helix_amd as hx
+    from helix_amd import closure as cl
+    steps = 2000
+    train_sets = [cl.Params.draw(7000 + i) for i in range(128)]
+    X, Y = cl.training_pairs(train_sets, keep_tier=2, deep_tier=6, steps=steps, every=100)
+    mlp = cl.MLP(X.shape[1], 256, 256, 2, seed=11)
+    losses = mlp.train(X, Y, epochs=20, batch=512, lr=1e-3, seed=3)
+    rows = []
+    for seed in (9000, 9001, 9002, 9003, 9004, 9005):
+        p = cl.Params.draw(seed)
+        deep = hx.Plan(p.model(6, "fp64"))
+        deep.set_initial()
+        t0 = time.perf_counter()
+        deep.step(steps)
+        t_deep = time.perf_counter() - t0
+        ref = deep.tier0()
+        deep.close()
+        s = cl.LearnedClosureSolver(p, mlp, keep_tier=2, precision="fp32")
+        s.enable(False)
+        plain = s.run(steps)
+        s.enable(True)
+        s.plan.set_initial()
+        t0 = time.perf_counter()
+        s.plan.step(steps)
+        t_learned = time.perf_counter() - t0
+        learned = s.plan.tier0()
+        rows.append(dict(seed=seed, err_plain_L2=float(np.max(np.abs(plain - ref))),
+                         err_learned_L2=float(np.max(np.abs(learned - ref))), t_deep_s=t_deep, t_learned_s=t_learned,
+                         n_ado_learned=s.model.H, n_ado_deep=3003))
+        s.close()
+    t_l = sum(r["t_learned_s"] for r in rows)
+    t_d = sum(r["t_deep_s"] for r in rows)
+    line = {
+        "metric": "ADO-steps/sec on 1 MI355X", "value": rows[0]["n_ado_learned"] * steps * len(rows) / t_l,
+        "unit": "ADO-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": t_l * 1e3 / (steps * len(rows)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded spin-glass parameter sets; 128 for training, 6 held out)",
+        "config": {"workload": "learned_closure_n16_L2", "spins": 4, "N": 16, "kept_tiers": "0..2",
+                   "reference_tiers": "0..6", "train_loss_first_last": [float(losses[0]), float(losses[-1])]},
+        "time_to_solution_x": t_d / t_l,
+        "mean_err_plain_L2": float(np.mean([r["err_plain_L2"] for r in rows])),
+        "mean_err_learned_L2": float(np.mean([r["err_learned_L2"] for r in rows])),
+        "held_out": rows,
+    }
+    print(json.dumps(line), flush=True)
+    mlp.close()
+    return 0
+
+
+def bench_legacy(args):
+    """The reference's compiled legacy spin glass (N = 1024, one bath with bit-flip coupling, K = 2,
+    JMax = 3 -> 10 ADOs, dt 0.1, Taylor order 4), the only workload the reference publishes a time for."""
+    import helix_amd as hx
+    m = hx.Model.legacy(N=1024, precision=args.precision)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(args.warmup)
+    t0 = time.perf_counter()
+    plan.step(args.steps)
+    sec = time.perf_counter() - t0
+    stage_ms, _ = plan.time_steps(max(1, args.timed_steps))
+    line = {"metric": "ADO-steps/sec on 1 MI355X", "value": m.H * args.steps / sec, "unit": "ADO-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3 / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
+            "data": "legacy compiled model", "config": {"workload": "legacy_spin_glass_n1024", "n_ado": m.H,
+                                                        "precision": args.precision},
+            "stage_ms": [float(x) for x in stage_ms], "launches_per_step": plan.info()["launches_per_step"]}
+    print(json.dumps(line), flush=True)
+    plan.close()
+    return 0
 
 
 def bench_batched(args):
@@ -116,19 +253,27 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="spin_glass_n64_L5", choices=sorted(WORKLOADS) + ["batched"])
+    ap.add_argument("--workload", default="spin_glass_n64_L5", choices=sorted(WORKLOADS) + ["batched", "learned", "legacy"])
     ap.add_argument("--batch-count", type=int, default=512, help="instances of the batched workload")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "split_fp32"])
     ap.add_argument("--timed-steps", type=int, default=20, help="steps timed per stage launch with HIP events")
     ap.add_argument("--dense-hamiltonian", action="store_true",
                     help="route -i[H, rho] through the dense MFMA commutator (N = 16, 32, 64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP32 / split-FP32 / batched lines measured beside the primary workload")
+    ap.add_argument("--form-steps", type=int, default=500, help="timed steps of each single-problem form")
+    ap.add_argument("--batch-steps", type=int, default=30, help="timed steps of each batched form")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args(argv)
 
     if args.workload.startswith("batched"):
         return bench_batched(args)
+    if args.workload == "learned":
+        return bench_learned(args)
+    if args.workload == "legacy":
+        return bench_legacy(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
@@ -184,6 +329,13 @@ def main(argv=None):
                      "stage_ms": [float(x) for x in stage_ms]},
         "launches_per_step": info["launches_per_step"],
     }
+    traffic = _profiled_traffic(args.workload, args.precision)
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic
+        line["roofline"]["traffic_source"] = "profiles/stage_traffic_spin_glass_n64_L5.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+    if not args.no_forms and args.workload == "spin_glass_n64_L5" and args.precision == "fp64" \
+            and not args.dense_hamiltonian:
+        line["forms"] = measure_forms(args, value)
     if args.copy_probe_mib:
         line["copy_probe_bytes"] = copy_probe(args.copy_probe_mib << 20)
     if not args.no_cpu_baseline:
