@@ -9,8 +9,9 @@ synthetic seeded couplings (seed 1).  One "step" advances every ADO by one full 
 value = ADO-steps/s = n_ado * steps / wall time of the timed steps (state resident in HBM).
 roofline: the fused stage kernel, algorithmic bytes 14 N^2 s per ADO-step (s = bytes per complex),
 divided over the m stage launches of a step, against the launch durations measured with HIP events on
-the plan's stream.  cpu_baseline: the FP64 C oracle (OpenMP) timed on this host for one step of the
-same workload.
+the plan's stream.  The device stores Hermitian ADOs in N(N+1)/2 elements, so the bytes it must move are
+(N+1)/(2N) of that figure (reported as roofline.stored_fraction).  cpu_baseline: the FP64 C oracle
+(OpenMP) timed on this host for one step of the same workload.
 """
 from __future__ import annotations
 
@@ -56,14 +57,16 @@ def cpu_baseline(spec, steps: int = 1):
             "sample": f"{steps} Taylor-4 step(s) of the full workload ({p.H} ADOs, N={p.N}) in {sec:.2f} s"}
 
 
-def _profiled_traffic(workload, precision):
+def _profiled_traffic(workload, precision, layout):
     """Mean HBM bytes per stage launch from the committed rocprofv3 counter summary, when it covers this
-    workload and precision."""
+    workload, precision and storage layout (tile kernel)."""
     path = os.path.join(ROOT, "profiles", "stage_traffic_spin_glass_n64_L5.json")
     if workload != "spin_glass_n64_L5" or not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
+    if not d.get("layout", "full").startswith(layout):
+        return None
     t = d.get("traffic", {}).get(precision)
     return None if t is None else t["mean_bytes_per_launch"]
 
@@ -199,9 +202,14 @@ def bench_legacy(args):
     plan.step(args.steps)
     sec = time.perf_counter() - t0
     stage_ms, _ = plan.time_steps(max(1, args.timed_steps))
-    line = {"metric": "ADO-steps/sec on 1 MI355X", "value": m.H * args.steps / sec, "unit": "ADO-steps/s", "n_gpus": 1,
+    value = m.H * args.steps / sec
+    # the reference's published legacy time (RTX 4070 SUPER, FP32): 85.718 ms per step of its 10 ADOs
+    ref_value = 10 / 85.718e-3
+    line = {"metric": "ADO-steps/sec on 1 MI355X", "value": value, "unit": "ADO-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3 / args.steps, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
+            "scaling": "weak", "vs_baseline": value / ref_value if args.precision == "fp32" else None,
+            "baseline_note": "reference FP32 legacy profile on an RTX 4070 SUPER: 85.718 ms/step = 116.7 ADO-steps/s",
+            "dtype": "f64" if args.precision == "fp64" else "f32",
             "data": "legacy compiled model", "config": {"workload": "legacy_spin_glass_n1024", "n_ado": m.H,
                                                         "precision": args.precision},
             "stage_ms": [float(x) for x in stage_ms], "launches_per_step": plan.info()["launches_per_step"]}
@@ -306,6 +314,7 @@ def main(argv=None):
     mean_launch_ms = float(sum(stage_ms) / len(stage_ms))
     achieved = per_launch / (mean_launch_ms * 1e-3) / 1e9
     info = plan.info()
+    layout = plan.layout()
     line = {
         "metric": "ADO-steps/sec on 1 MI355X",
         "value": value,
@@ -321,15 +330,18 @@ def main(argv=None):
         "data": "synthetic (seeded spin-glass couplings, rho_0 = |up..up><up..up|)",
         "config": {"workload": args.workload, "spins": spins, "N": model.N, "n_ado": model.H, "modes": model.M,
                    "max_tier": L, "dt": dt, "order": model.order, "precision": args.precision, "seed": seed,
-                   "hamiltonian": "dense (MFMA)" if args.dense_hamiltonian else "spin-structured (diag + flips)"},
+                   "hamiltonian": "dense (MFMA)" if args.dense_hamiltonian else "spin-structured (diag + flips)",
+                   "layout": layout},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_stage (fused Liouville RHS + Taylor epilogue)",
+                     "kernel": ("k_stage_dense" if args.dense_hamiltonian else "k_stage_tile")
+                     + " (fused Liouville RHS + Taylor epilogue)",
                      "alg_bytes_per_launch": per_launch, "mean_launch_ms": mean_launch_ms,
-                     "stage_ms": [float(x) for x in stage_ms]},
+                     "stage_ms": [float(x) for x in stage_ms],
+                     "stored_fraction": (model.N + 1) / (2 * model.N) if layout == "hermitian" else 1.0},
         "launches_per_step": info["launches_per_step"],
     }
-    traffic = _profiled_traffic(args.workload, args.precision)
+    traffic = None if args.dense_hamiltonian else _profiled_traffic(args.workload, args.precision, layout)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic
         line["roofline"]["traffic_source"] = "profiles/stage_traffic_spin_glass_n64_L5.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)"

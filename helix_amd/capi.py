@@ -17,6 +17,9 @@ HOST_LIB = os.path.join(PKG, "libhelix.so")
 
 FP64, FP32, SPLIT_FP32 = 0, 1, 2
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "split_fp32": SPLIT_FP32}
+# helix_hip_layout: how each ADO block is stored on the device
+LAYOUTS = {"auto": 0, "full": 1, "hermitian": 2}
+LAYOUT_NAMES = {1: "full", 2: "hermitian"}
 
 # every symbol include/helix_hip.h and include/helix_setup.h declare
 HIP_SYMBOLS = [
@@ -25,6 +28,7 @@ HIP_SYMBOLS = [
     "helix_hip_last_error", "helix_hip_batch_create", "helix_hip_batch_set_initial", "helix_hip_batch_step",
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
+    "helix_hip_plan_layout", "helix_hip_batch_layout",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_use_dense_hamiltonian",
@@ -49,7 +53,7 @@ class Problem(C.Structure):
         ("n_baths", C.c_int32), ("up", _ip), ("down", _ip), ("mode_bath", _ip), ("tier", _ip),
         ("down_comm", _dp), ("down_anti", _dp), ("self_coef", _dp), ("xi", _dp), ("xi_counter", _dp),
         ("hamiltonian", SpinOperator), ("hamiltonian_dense", _dp), ("couplings", C.POINTER(SpinOperator)),
-        ("precision", C.c_int32), ("order", C.c_int32), ("dt", C.c_double),
+        ("precision", C.c_int32), ("order", C.c_int32), ("dt", C.c_double), ("layout", C.c_int32),
     ]
 
 
@@ -96,6 +100,8 @@ def hip():
                                            C.POINTER(C.c_int)]
         L.helix_hip_batch_destroy.argtypes = [vp]
         L.helix_hip_batch_destroy.restype = None
+        L.helix_hip_plan_layout.argtypes = [vp, _ip]
+        L.helix_hip_batch_layout.argtypes = [vp, _ip]
         _hip = L
     return _hip
 
@@ -244,15 +250,30 @@ class Model:
             pass
 
 
+def _with_layout(model, layout: str) -> Problem:
+    """A copy of the model's problem descriptor with the storage layout set (the tables it points to stay
+    owned by the model; a plan copies them at creation)."""
+    desc = Problem()
+    C.memmove(C.byref(desc), model.problem_ptr(), C.sizeof(Problem))
+    desc.layout = LAYOUTS[layout]
+    return desc
+
+
 class Plan:
     """A device-resident HEOM propagator (helix_hip_plan)."""
 
-    def __init__(self, model: Model, device: int = 0):
+    def __init__(self, model: Model, device: int = 0, layout: str = "auto"):
         self.model = model
         self.N, self.H = model.N, model.H
         self._p = C.c_void_p()
-        _check(hip().helix_hip_plan_create(C.cast(model.problem_ptr(), C.c_void_p), device, C.byref(self._p)),
+        desc = _with_layout(model, layout)
+        _check(hip().helix_hip_plan_create(C.cast(C.pointer(desc), C.c_void_p), device, C.byref(self._p)),
                "helix_hip_plan_create")
+
+    def layout(self) -> str:
+        v = C.c_int32()
+        _check(hip().helix_hip_plan_layout(self._p, C.byref(v)), "helix_hip_plan_layout")
+        return LAYOUT_NAMES[v.value]
 
     def set_initial(self, rho0: np.ndarray | None = None):
         r = self.model.initial_rho() if rho0 is None else np.ascontiguousarray(rho0, np.complex128)
@@ -316,13 +337,19 @@ BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
 class Batch:
     """Many independent problems advanced together: one launch per Taylor stage for all of them."""
 
-    def __init__(self, models, device: int = 0, graph: bool = True, autotune: bool = True):
+    def __init__(self, models, device: int = 0, graph: bool = True, autotune: bool = True, layout: str = "auto"):
         self.models = list(models)
-        arr = (C.c_void_p * len(self.models))(*[C.cast(m.problem_ptr(), C.c_void_p) for m in self.models])
+        descs = [_with_layout(m, layout) for m in self.models]
+        arr = (C.c_void_p * len(descs))(*[C.cast(C.pointer(d), C.c_void_p) for d in descs])
         self._b = C.c_void_p()
         flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
         _check(hip().helix_hip_batch_create(arr, len(self.models), device, flags, C.byref(self._b)),
                "helix_hip_batch_create")
+
+    def layout(self) -> str:
+        v = C.c_int32()
+        _check(hip().helix_hip_batch_layout(self._b, C.byref(v)), "helix_hip_batch_layout")
+        return LAYOUT_NAMES[v.value]
 
     def set_initial(self, index: int = -1, rho0: np.ndarray | None = None):
         r = None if rho0 is None else np.ascontiguousarray(rho0, np.complex128)

@@ -241,7 +241,7 @@ class LearnedClosureSolver:
     def __init__(self, params: Params, mlp: MLP, keep_tier: int = 2, precision: str = "fp32", dt: float = 0.01):
         self.params, self.mlp, self.keep_tier, self.dt = params, mlp, keep_tier, dt
         self.model = params.model(keep_tier, precision, dt)
-        self.plan = Plan(self.model)
+        self.plan = Plan(self.model, layout="full")  # the rates need not be conjugate-symmetric
         rows = self.model.rows()
         self.ados = np.ascontiguousarray(np.nonzero(rows.sum(1) == keep_tier)[0], np.int32)
         self.extra = np.ascontiguousarray(params.extra(rows[self.ados], keep_tier), np.float32)

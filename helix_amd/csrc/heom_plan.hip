@@ -54,6 +54,25 @@ struct C2t<float> {
 	using type = float2;
 };
 
+// Hermitian layout (include/helix_hip.h): stored index of full element (i, j) of an N x N block, and
+// whether the stored value must be conjugated.  Main [N/2][N]: (d, i) = rho[i][(i + d) mod N] for
+// d < N/2; tail [N/2]: t = rho[t][t + N/2]; other elements are conjugates of stored ones.
+__host__ __device__ inline int herm_index(int i, int j, int logN, bool& conj)
+{
+	const int N = 1 << logN;
+	const int d = (j - i) & (N - 1);
+	if(d < N / 2) {
+		conj = false;
+		return (d << logN) | i;
+	}
+	if(d > N / 2) {
+		conj = true;
+		return ((N - d) << logN) | j;
+	}
+	conj = i >= N / 2;
+	return N * N / 2 + (conj ? j : i);
+}
+
 // acc += w * x for a complex weight w given in FP64.  SPLIT (FP32 only) applies w as the two-term
 // sum w_hi + w_lo of FP32 values so operator values keep ~FP64 accuracy while products and
 // accumulation stay in FP32.
@@ -268,17 +287,21 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 	}
 }
 
-// LDS-tiled stage for diagonal couplings and N <= 64 (LOGN 3..6).
-// A workgroup owns a tile of 4096 elements = A = 4096/N^2 whole ADO blocks; each ADO is handled by
-// THREADS/A threads with 4096/THREADS elements per thread (element e = lane_in_ado + (THREADS/A) r, so
-// every global access is a contiguous 16-byte-per-lane row).  The tile's own blocks are staged once in
-// LDS, so every Hamiltonian bit-flip gather (rows i^m, columns j^m) is served from LDS.  The per-ADO
-// neighbour table (n+e_k, n-e_k) is copied to LDS and walked entry by entry; each valid neighbour issues
-// all of its loads before any arithmetic.  Workgroups are renumbered so each XCD (round-robin dispatch
-// over 8 XCDs) sweeps one contiguous range of ADOs: the near neighbours in lexicographic order
-// (n +- e_k for the trailing modes) then hit that XCD's L2.
+// LDS-tiled stage for diagonal couplings, N <= 1024 (LOGN 3..10).
+// A workgroup owns a tile of 4096 elements = A = 4096/N^2 whole ADO blocks (N <= 64) or one row band of
+// one block (N > 64, TPA bands per block); each ADO is handled by TA = THREADS/A threads, element
+// e = lane_in_ado + TA r, so every global access is a contiguous 16-byte-per-lane row.  The tile's own
+// blocks are staged once in LDS, so every Hamiltonian bit-flip gather (rows i^m, columns j^m) is served
+// from LDS.  The per-ADO neighbour table (n+e_k, n-e_k) is copied to LDS and walked entry by entry; each
+// valid neighbour issues all of its loads before any arithmetic.  Workgroups are renumbered so each XCD
+// (round-robin dispatch over 8 XCDs) sweeps one contiguous range of ADOs: the near neighbours in
+// lexicographic order (n +- e_k for the trailing modes) then hit that XCD's L2.
 // SZ: every coupling is alpha_b (2 n_{s_b} - 1): commutator / anticommutator weights come from bits
 // (d_b = v_i - v_j and s_b = v_i + v_j, one of them zero) and the Sigma + Xi weight from selfx[i^j].
+// HERM: the Hermitian layout of include/helix_hip.h (N(N+1)/2 stored elements per ADO, N <= 64).  The
+// stored elements are loaded once and written to both (i, j) and (j, i) of the full block in LDS; the
+// tile computes only the stored elements (every term of L is elementwise in the neighbours, and the
+// Hamiltonian gathers come from the full LDS block), so loads, stores and neighbour reads all halve.
 constexpr int kTile = 4096;  // elements per workgroup
 constexpr int kTileMaxModes = 16;
 constexpr int kXcds = 8;
@@ -311,18 +334,47 @@ struct TileLds {
 	int* nb;        // [A][2 kTileMaxModes]
 };
 
-template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ>
+// Register slot r of a tile thread (lt = lane within its ADO): the stored element q within the ADO and
+// its full-block row i and column j.  Full layout: q = ebase + lt + TA r = i N + j.  Hermitian layout:
+// slots r < EPT are main elements q = lt + TA r = d N + i, j = (i + d) mod N; slot EPT is the tail
+// element q = N^2/2 + lt (i = lt, j = lt + N/2), held by threads lt < N/2.
+template <int LOGN, int TA, int EPT, bool HERM>
+__device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int& i, int& j)
+{
+	constexpr int N = 1 << LOGN;
+	if constexpr(HERM) {
+		if(r < EPT) {
+			q = lt + TA * r;
+			i = q & (N - 1);
+			j = (i + (q >> LOGN)) & (N - 1);
+		} else {
+			q = N * N / 2 + lt;
+			i = lt;
+			j = lt + N / 2;
+		}
+	} else {
+		q = ebase + lt + TA * r;
+		i = q >> LOGN;
+		j = q & (N - 1);
+	}
+}
+
+template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
 __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, const TileLds<T>& lds)
 {
 	using V2 = typename C2t<T>::type;
-	constexpr int EPT = kTile / THREADS;
 	constexpr int N = 1 << LOGN;
 	constexpr int LOGE = 2 * LOGN;
 	constexpr int E = 1 << LOGE;
+	static_assert(!HERM || E <= kTile, "the Hermitian layout tiles whole ADO blocks (N <= 64)");
 	constexpr int TPA = (E > kTile) ? E / kTile : 1;  // tiles per ADO (N > 64: row tiles of one block)
 	constexpr int A = (E >= kTile) ? 1 : kTile / E;   // ADOs per workgroup (N < 64)
-	constexpr int TE = (E >= kTile) ? kTile : E;      // elements of one ADO held by the tile
+	constexpr int TE = (E >= kTile) ? kTile : E;      // full-block elements of one ADO held in LDS
 	constexpr int TA = THREADS / A;                   // threads per ADO
+	constexpr int EPT = (HERM ? TE / 2 : TE) / TA;    // main elements per thread
+	constexpr int RT = HERM ? EPT + 1 : EPT;          // register slots per thread (+ the tail element)
+	constexpr size_t PE = HERM ? (static_cast<size_t>(E) + N) / 2 : static_cast<size_t>(E);  // stored per ADO
+	static_assert(!HERM || TA >= N / 2, "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
 	double2* __restrict__ s_selfx = lds.selfx;
@@ -336,6 +388,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	const int lt = tid - slot * TA;
 	const int n = (group / TPA) * A + slot;  // this thread's ADO (uniform over TA threads)
 	const bool live = n < H;
+	const bool tail_ok = HERM && lt < N / 2;
+	// slot r holds an element: every main slot, the tail slot only for lt < N/2
+#define HX_SLOT_LIVE(r) (!HERM || (r) < EPT || tail_ok)
 
 	if constexpr(SZ) {
 		for(int q = tid; q < N; q += THREADS) s_selfx[q] = a.selfx[q];
@@ -352,23 +407,41 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		}
 		s_nb[s][k] = idx;
 	}
+	if constexpr(HERM) {
+		if(live) {
 #pragma unroll
-	for(int r = 0; r < EPT; ++r) {
-		const int le = lt + TA * r;
-		s_blk[slot * TE + le] = live ? cur[(static_cast<size_t>(n) << LOGE) + ebase + le] : V2{T(0), T(0)};
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				int q, i, j;
+				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+				const V2 v = cur[static_cast<size_t>(n) * PE + q];
+				s_blk[slot * TE + ((i << LOGN) | j)] = v;
+				if(i != j) s_blk[slot * TE + ((j << LOGN) | i)] = V2{v.x, -v.y};
+			}
+		}
+	} else {
+#pragma unroll
+		for(int r = 0; r < EPT; ++r) {
+			const int le = lt + TA * r;
+			s_blk[slot * TE + le] = live ? cur[(static_cast<size_t>(n) << LOGE) + ebase + le] : V2{T(0), T(0)};
+		}
 	}
 	__syncthreads();
 	if(!live) return;
 
-	T ar[EPT], ai[EPT];
+	T ar[RT], ai[RT];
 	const V2* __restrict__ blk = s_blk + slot * TE;
-	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);
+	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);  // full layout, N > 64 only
 	// -i[H, rho_n] from LDS
 #pragma unroll
-	for(int r = 0; r < EPT; ++r) {
-		const int le = lt + TA * r;
-		const int e = ebase + le;
-		const int i = e >> LOGN, j = e & (N - 1);
+	for(int r = 0; r < RT; ++r) {
+		ar[r] = T(0);
+		ai[r] = T(0);
+		if(!HX_SLOT_LIVE(r)) continue;
+		int q, i, j;
+		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+		const int e = (i << LOGN) | j;
+		const int le = e - ebase;
 		const V2 c = blk[le];
 		T hr = T(0), hi = T(0);
 		rmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], c.x, c.y);
@@ -390,10 +463,16 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		const bool down = q >= M;
 		const int k = down ? q - M : q;
 		const int b = a.mode_bath[k];
-		const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
-		V2 x[EPT];
+		const V2* __restrict__ src = cur + static_cast<size_t>(idx) * PE;
+		V2 x[RT];
 #pragma unroll
-		for(int r = 0; r < EPT; ++r) x[r] = src[ebase + lt + TA * r];
+		for(int r = 0; r < RT; ++r) {
+			x[r] = V2{T(0), T(0)};
+			if(!HX_SLOT_LIVE(r)) continue;
+			int qe, i, j;
+			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+			x[r] = src[qe];
+		}
 		if constexpr(SZ) {
 			// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
 			// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
@@ -406,9 +485,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				wanti = a.danti[n * M + k].x;
 			}
 #pragma unroll
-			for(int r = 0; r < EPT; ++r) {
-				const int e = ebase + lt + TA * r;
-				const int bi = (e >> LOGN) & mask, bj = e & mask;
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				int qe, i, j;
+				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+				const int bi = i & mask, bj = j & mask;
 				const bool differ = bi != bj;
 				const double sg = bi ? two_alpha : -two_alpha;
 				const double w = sg * (differ ? wcomm : wanti);
@@ -425,9 +506,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			}
 			const double* __restrict__ vb = s_v + b * N;
 #pragma unroll
-			for(int r = 0; r < EPT; ++r) {
-				const int e = ebase + lt + TA * r;
-				const double vi = vb[e >> LOGN], vj = vb[e & (N - 1)];
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				int qe, i, j;
+				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+				const double vi = vb[i], vj = vb[j];
 				const double d = vi - vj, s = vi + vj;
 				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
 			}
@@ -440,10 +523,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
 #pragma unroll 2
-	for(int r = 0; r < EPT; ++r) {
-		const int le = lt + TA * r;
-		const int e = ebase + le;
-		const int i = e >> LOGN, j = e & (N - 1);
+	for(int r = 0; r < RT; ++r) {
+		if(!HX_SLOT_LIVE(r)) continue;
+		int qe, i, j;
+		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+		const int e = (i << LOGN) | j;
 		double wr = sf.x, wi = sf.y;
 		if constexpr(SZ) {
 			const double2 x = s_selfx[i ^ j];
@@ -457,25 +541,27 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				wi -= a.xi[b].y * d * d + a.xic[b].y * s * d;
 			}
 		}
-		if(a.kappa) {
-			const int sl = a.kslot[n];
-			if(sl >= 0) {
-				const double2 kk = a.kappa[(static_cast<size_t>(sl) << LOGE) + e];
-				wr += kk.x;
-				wi += kk.y;
+		if constexpr(!HERM) {
+			if(a.kappa) {
+				const int sl = a.kslot[n];
+				if(sl >= 0) {
+					const double2 kk = a.kappa[(static_cast<size_t>(sl) << LOGE) + e];
+					wr += kk.x;
+					wi += kk.y;
+				}
 			}
 		}
-		const V2 c = blk[le];
+		const V2 c = blk[e - ebase];
 		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c.x, c.y);
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
-		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
+		const size_t g = static_cast<size_t>(n) * PE + qe;
 		if constexpr(!LAST) {
 			out[g] = V2{tr, ti};
 		} else {
 			const int nt = a.ntprev;
 			if(a.rho0_f64 != nullptr && n == 0) {
 				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
-				double2 rv = r0[e];
+				double2 rv = r0[qe];
 				for(int q = 0; q + 1 < nt; ++q) {
 					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
 					rv.x += static_cast<double>(t.x);
@@ -487,7 +573,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				}
 				rv.x += static_cast<double>(tr);
 				rv.y += static_cast<double>(ti);
-				r0[e] = rv;
+				r0[qe] = rv;
 				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
 			} else {
 				V2 rv = (nt == 0) ? c : static_cast<const V2*>(a.rho)[g];
@@ -506,9 +592,10 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			}
 		}
 	}
+#undef HX_SLOT_LIVE
 }
 
-template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ>
+template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
 k_stage_tile(const StageArgs a, int ngroups)
 {
@@ -524,7 +611,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
 	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(group >= ngroups) return;
-	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb});
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb});
 }
 
 // Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
@@ -533,7 +620,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 // instance's tiles on one XCD.
 constexpr int kBatchMaxLogN = 6;
 
-template <typename T, bool SPLIT, bool LAST, int THREADS>
+template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
 k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ tasks, int ntasks)
 {
@@ -549,10 +636,10 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	const StageArgs& a = table[tk.x];
 	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb};
 	switch(a.logN) {
-	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true>(a, tk.y, lds); break;
-	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true>(a, tk.y, lds); break;
-	case 5: tile_body<T, SPLIT, LAST, 5, THREADS, true>(a, tk.y, lds); break;
-	default: tile_body<T, SPLIT, LAST, 6, THREADS, true>(a, tk.y, lds); break;
+	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
+	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
+	case 5: tile_body<T, SPLIT, LAST, 5, THREADS, true, HERM>(a, tk.y, lds); break;
+	default: tile_body<T, SPLIT, LAST, 6, THREADS, true, HERM>(a, tk.y, lds); break;
 	}
 }
 
@@ -826,29 +913,34 @@ __global__ __launch_bounds__(kBlock) void k_kappa_store(size_t rows, const float
 		kappa[r] = make_double2(static_cast<double>(y[2 * r]), static_cast<double>(y[2 * r + 1]));
 }
 
-// tier-0 block -> trace buffers (FP64): full block and/or real diagonal
+// tier-0 block -> trace buffers (FP64): full block and/or real diagonal; herm: the block is stored in
+// the Hermitian layout
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, double* rho_out,
-	double* diag_out)
+__global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const double* rho0_f64, int logN, int herm,
+	double* rho_out, double* diag_out)
 {
 	using V2 = typename C2t<T>::type;
 	const uint32_t N = 1u << logN;
 	const uint32_t E = N * N;
 	for(uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < E; e += gridDim.x * kBlock) {
+		const int i = static_cast<int>(e >> logN), j = static_cast<int>(e & (N - 1));
+		bool conj = false;
+		const uint32_t src = herm ? static_cast<uint32_t>(herm_index(i, j, logN, conj)) : e;
 		double re, im;
 		if(rho0_f64) {
-			re = rho0_f64[2 * e];
-			im = rho0_f64[2 * e + 1];
+			re = rho0_f64[2 * src];
+			im = rho0_f64[2 * src + 1];
 		} else {
-			const V2 v = static_cast<const V2*>(tier0)[e];
+			const V2 v = static_cast<const V2*>(tier0)[src];
 			re = static_cast<double>(v.x);
 			im = static_cast<double>(v.y);
 		}
+		if(conj) im = -im;
 		if(rho_out) {
 			rho_out[2 * e] = re;
 			rho_out[2 * e + 1] = im;
 		}
-		if(diag_out && (e >> logN) == (e & (N - 1))) diag_out[e >> logN] = re;
+		if(diag_out && i == j) diag_out[i] = re;
 	}
 }
 
@@ -864,8 +956,10 @@ struct helix_hip_plan {
 	bool flip = false;
 	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
 	bool dense = false;       // dense Hamiltonian on MFMA (N in {16, 32, 64})
+	bool herm = false;        // Hermitian layout: N(N+1)/2 stored elements per ADO
 	size_t esize = 16;        // bytes per complex element
-	size_t elems = 0;         // H * N * N
+	size_t pe = 0;            // stored elements per ADO (N^2, or N(N+1)/2 in the Hermitian layout)
+	size_t elems = 0;         // H * pe
 	void* rho = nullptr;
 	void* t[kMaxOrder + 1] = {};
 	double* rho0_f64 = nullptr;
@@ -935,6 +1029,8 @@ int validate(const helix_hip_problem* pr)
 	if(!(pr->dt > 0.0)) return fail(HELIX_HIP_EINVAL, "dt must be positive");
 	if(pr->precision < HELIX_HIP_FP64 || pr->precision > HELIX_HIP_SPLIT_FP32)
 		return fail(HELIX_HIP_EINVAL, "unknown precision");
+	if(pr->layout < HELIX_HIP_LAYOUT_AUTO || pr->layout > HELIX_HIP_LAYOUT_HERMITIAN)
+		return fail(HELIX_HIP_EINVAL, "unknown layout");
 	if(pr->hamiltonian_dense) {
 		if(pr->n != 16 && pr->n != 32 && pr->n != 64)
 			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports N = 16, 32, 64");
@@ -973,7 +1069,7 @@ unsigned grid_for(size_t total)
 	return static_cast<unsigned>(g ? g : 1);
 }
 
-template <typename T, bool SPLIT, int LOGN, int THREADS, bool SZ>
+template <typename T, bool SPLIT, int LOGN, int THREADS, bool SZ, bool HERM>
 void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
 {
 	constexpr int E = 1 << (2 * LOGN);
@@ -982,10 +1078,10 @@ void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
 	const int ngroups = ((p->H + A - 1) / A) * TPA;
 	const unsigned grid = static_cast<unsigned>(((ngroups + kXcds - 1) / kXcds) * kXcds);
 	if(last)
-		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN, THREADS, SZ>), dim3(grid), dim3(THREADS), 0, p->stream,
-			a, ngroups);
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, true, LOGN, THREADS, SZ, HERM>), dim3(grid), dim3(THREADS), 0,
+			p->stream, a, ngroups);
 	else
-		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, false, LOGN, THREADS, SZ>), dim3(grid), dim3(THREADS), 0,
+		hipLaunchKernelGGL((k_stage_tile<T, SPLIT, false, LOGN, THREADS, SZ, HERM>), dim3(grid), dim3(THREADS), 0,
 			p->stream, a, ngroups);
 }
 
@@ -993,10 +1089,14 @@ void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
 template <typename T, bool SPLIT, int LOGN, bool SZ>
 void launch_tile_sz(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	if constexpr(sizeof(T) == 8)
-		launch_tile_v<T, SPLIT, LOGN, 512, SZ>(p, a, last);
-	else
-		launch_tile_v<T, SPLIT, LOGN, 1024, SZ>(p, a, last);
+	constexpr int THREADS = (sizeof(T) == 8) ? 512 : 1024;
+	if constexpr(LOGN <= 6) {
+		if(p->herm) {
+			launch_tile_v<T, SPLIT, LOGN, THREADS, SZ, true>(p, a, last);
+			return;
+		}
+	}
+	launch_tile_v<T, SPLIT, LOGN, THREADS, SZ, false>(p, a, last);
 }
 
 template <typename T, bool SPLIT, int LOGN>
@@ -1107,34 +1207,136 @@ int launch_error(const char* what)
 	return 0;
 }
 
+// One N x N complex block (interleaved doubles) into the Hermitian layout; false when some
+// |rho[j][i] - conj(rho[i][j])| exceeds tol.
+bool pack_block(const double* full, int logN, double* packed, double tol)
+{
+	const int N = 1 << logN;
+	for(int i = 0; i < N; ++i)
+		for(int j = i + 1; j < N; ++j) {
+			const double* a = full + 2 * (static_cast<size_t>(i) * N + j);
+			const double* b = full + 2 * (static_cast<size_t>(j) * N + i);
+			if(std::fabs(a[0] - b[0]) > tol || std::fabs(a[1] + b[1]) > tol) return false;
+		}
+	for(int i = 0; i < N; ++i)
+		for(int j = 0; j < N; ++j) {
+			bool conj = false;
+			const int q = herm_index(i, j, logN, conj);
+			if(conj) continue;
+			packed[2 * q] = full[2 * (static_cast<size_t>(i) * N + j)];
+			packed[2 * q + 1] = full[2 * (static_cast<size_t>(i) * N + j) + 1];
+		}
+	return true;
+}
+
+void unpack_block(const double* packed, int logN, double* full)
+{
+	const int N = 1 << logN;
+	for(int i = 0; i < N; ++i)
+		for(int j = 0; j < N; ++j) {
+			bool conj = false;
+			const int q = herm_index(i, j, logN, conj);
+			full[2 * (static_cast<size_t>(i) * N + j)] = packed[2 * q];
+			full[2 * (static_cast<size_t>(i) * N + j) + 1] = conj ? -packed[2 * q + 1] : packed[2 * q + 1];
+		}
+}
+
+// A full [H][N][N] state into the stored layout (FP64, interleaved); EINVAL for a non-Hermitian state
+// in the Hermitian layout.  Tolerance: 1e-12 of the state's largest component.
+int to_stored(const double* state, int H, int logN, bool herm, std::vector<double>& st)
+{
+	const size_t n2 = static_cast<size_t>(1) << (2 * logN);
+	if(!herm) {
+		st.assign(state, state + static_cast<size_t>(H) * n2 * 2);
+		return 0;
+	}
+	const size_t pe = (n2 + (static_cast<size_t>(1) << logN)) / 2;
+	double mx = 0.0;
+	for(size_t q = 0; q < static_cast<size_t>(H) * n2 * 2; ++q) mx = std::max(mx, std::fabs(state[q]));
+	st.assign(static_cast<size_t>(H) * pe * 2, 0.0);
+	for(int h = 0; h < H; ++h)
+		if(!pack_block(state + static_cast<size_t>(h) * n2 * 2, logN, st.data() + static_cast<size_t>(h) * pe * 2, 1e-12 * mx))
+			return fail(HELIX_HIP_EINVAL, "state is not Hermitian (ADO " + std::to_string(h) +
+				"); the Hermitian layout needs Hermitian ADOs -- create the plan with HELIX_HIP_LAYOUT_FULL");
+	return 0;
+}
+
+void from_stored(const double* st, int H, int logN, bool herm, double* state)
+{
+	const size_t n2 = static_cast<size_t>(1) << (2 * logN);
+	if(!herm) {
+		std::memcpy(state, st, static_cast<size_t>(H) * n2 * 16);
+		return;
+	}
+	const size_t pe = (n2 + (static_cast<size_t>(1) << logN)) / 2;
+	for(int h = 0; h < H; ++h) unpack_block(st + static_cast<size_t>(h) * pe * 2, logN, state + static_cast<size_t>(h) * n2 * 2);
+}
+
+// device stored elements (state precision) -> FP64 interleaved
+int download_stored(const void* dev, size_t count, bool fp64, double* out)
+{
+	if(fp64) {
+		HX_HIP(hipMemcpy(out, dev, count * 16, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> f(count * 2);
+		HX_HIP(hipMemcpy(f.data(), dev, count * 8, hipMemcpyDeviceToHost));
+		for(size_t q = 0; q < count * 2; ++q) out[q] = static_cast<double>(f[q]);
+	}
+	return 0;
+}
+
+int upload_stored(void* dev, const double* st, size_t count, bool fp64)
+{
+	if(fp64) {
+		HX_HIP(hipMemcpy(dev, st, count * 16, hipMemcpyHostToDevice));
+	} else {
+		std::vector<float> f(count * 2);
+		for(size_t q = 0; q < count * 2; ++q) f[q] = static_cast<float>(st[q]);
+		HX_HIP(hipMemcpy(dev, f.data(), count * 8, hipMemcpyHostToDevice));
+	}
+	return 0;
+}
+
 int host_to_state(helix_hip_plan* p, const double* state)
 {
-	const size_t n2 = static_cast<size_t>(p->N) * p->N;
-	if(p->precision == HELIX_HIP_FP64) {
-		HX_HIP(hipMemcpy(p->rho, state, p->elems * 16, hipMemcpyHostToDevice));
-	} else {
-		std::vector<float> f(p->elems * 2);
-		for(size_t q = 0; q < p->elems * 2; ++q) f[q] = static_cast<float>(state[q]);
-		HX_HIP(hipMemcpy(p->rho, f.data(), p->elems * 8, hipMemcpyHostToDevice));
-		HX_HIP(hipMemcpy(p->rho0_f64, state, n2 * 16, hipMemcpyHostToDevice));
-	}
+	std::vector<double> st;
+	if(int rc = to_stored(state, p->H, p->logN, p->herm, st)) return rc;
+	const bool fp64 = p->precision == HELIX_HIP_FP64;
+	if(int rc = upload_stored(p->rho, st.data(), p->elems, fp64)) return rc;
+	if(!fp64) HX_HIP(hipMemcpy(p->rho0_f64, st.data(), p->pe * 16, hipMemcpyHostToDevice));
 	return 0;
 }
 
 int state_to_host(helix_hip_plan* p, const void* dev, double* out, bool tier0_master)
 {
-	if(p->precision == HELIX_HIP_FP64) {
-		HX_HIP(hipMemcpy(out, dev, p->elems * 16, hipMemcpyDeviceToHost));
-	} else {
-		std::vector<float> f(p->elems * 2);
-		HX_HIP(hipMemcpy(f.data(), dev, p->elems * 8, hipMemcpyDeviceToHost));
-		for(size_t q = 0; q < p->elems * 2; ++q) out[q] = static_cast<double>(f[q]);
-		if(tier0_master) {
-			const size_t n2 = static_cast<size_t>(p->N) * p->N;
-			HX_HIP(hipMemcpy(out, p->rho0_f64, n2 * 16, hipMemcpyDeviceToHost));
-		}
-	}
+	std::vector<double> st(p->elems * 2);
+	const bool fp64 = p->precision == HELIX_HIP_FP64;
+	if(int rc = download_stored(dev, p->elems, fp64, st.data())) return rc;
+	if(!fp64 && tier0_master) HX_HIP(hipMemcpy(st.data(), p->rho0_f64, p->pe * 16, hipMemcpyDeviceToHost));
+	from_stored(st.data(), p->H, p->logN, p->herm, out);
 	return 0;
+}
+
+// problem coefficients under which L maps Hermitian ADOs to Hermitian ADOs (see helix_hip_layout)
+bool preserves_hermiticity(const helix_hip_problem* pr)
+{
+	const size_t HM = static_cast<size_t>(pr->n_ado) * pr->n_modes;
+	for(size_t q = 0; q < HM; ++q)
+		if(pr->down_comm[2 * q] != 0.0 || pr->down_anti[2 * q + 1] != 0.0) return false;
+	for(int h = 0; h < pr->n_ado; ++h)
+		if(pr->self_coef[2 * h + 1] != 0.0) return false;
+	for(int b = 0; b < pr->n_baths; ++b)
+		if(pr->xi[2 * b + 1] != 0.0 || pr->xi_counter[2 * b] != 0.0) return false;
+	if(pr->hamiltonian_dense) {
+		const int N = pr->n;
+		for(int i = 0; i < N; ++i)
+			for(int j = 0; j < N; ++j) {
+				const double* a = pr->hamiltonian_dense + 2 * (static_cast<size_t>(i) * N + j);
+				const double* b = pr->hamiltonian_dense + 2 * (static_cast<size_t>(j) * N + i);
+				if(a[0] != b[0] || a[1] != -b[1]) return false;
+			}
+	}
+	return true;
 }
 
 }  // namespace
@@ -1168,7 +1370,6 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	p->precision = pr->precision;
 	p->dt = pr->dt;
 	p->esize = (pr->precision == HELIX_HIP_FP64) ? 16 : 8;
-	p->elems = static_cast<size_t>(p->H) << (2 * p->logN);
 	for(int b = 0; b < p->B; ++b)
 		if(pr->couplings[b].nflips > 0) p->flip = true;
 	{
@@ -1179,17 +1380,6 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
 
 	int rc = 0;
-	if((rc = plan_alloc(p, &p->rho, p->elems * p->esize))) return bail(rc);
-	const int nt = std::max(1, p->order - 1);
-	for(int q = 1; q <= nt; ++q)
-		if((rc = plan_alloc(p, &p->t[q], p->elems * p->esize))) return bail(rc);
-	if(p->precision != HELIX_HIP_FP64) {
-		void* r0 = nullptr;
-		if((rc = plan_alloc(p, &r0, static_cast<size_t>(p->N) * p->N * 16))) return bail(rc);
-		p->rho0_f64 = static_cast<double*>(r0);
-	}
-	if(p->flip && (rc = plan_alloc(p, &p->buf, p->elems * p->esize * p->B))) return bail(rc);
-
 	StageArgs& a = p->base;
 	std::memset(&a, 0, sizeof(a));
 	const size_t HM = static_cast<size_t>(p->H) * p->M;
@@ -1285,13 +1475,33 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	if(p->logN > 6 && !a.sz) p->tiled = false;  // see launch_tile
 	if(p->dense) p->tiled = false;
 	if(p->logN > 10) p->tiled = false;
+
+	// storage layout: Hermitian where the tile kernel runs whole blocks (N <= 64) and L preserves it
+	const bool can_herm = p->tiled && p->logN <= 6 && preserves_hermiticity(pr);
+	if(pr->layout == HELIX_HIP_LAYOUT_HERMITIAN && !can_herm)
+		return bail(fail(HELIX_HIP_EUNSUPPORTED, "Hermitian layout needs the tiled stage (diagonal couplings, N <= 64) "
+			"and Hermiticity-preserving coefficients"));
+	p->herm = can_herm && pr->layout != HELIX_HIP_LAYOUT_FULL;
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	p->pe = p->herm ? (n2 + p->N) / 2 : n2;
+	p->elems = static_cast<size_t>(p->H) * p->pe;
+	if((rc = plan_alloc(p, &p->rho, p->elems * p->esize))) return bail(rc);
+	const int nt = std::max(1, p->order - 1);
+	for(int q = 1; q <= nt; ++q)
+		if((rc = plan_alloc(p, &p->t[q], p->elems * p->esize))) return bail(rc);
+	if(p->precision != HELIX_HIP_FP64) {
+		void* r0 = nullptr;
+		if((rc = plan_alloc(p, &r0, p->pe * 16))) return bail(rc);
+		p->rho0_f64 = static_cast<double*>(r0);
+	}
+	if(p->flip && (rc = plan_alloc(p, &p->buf, p->elems * p->esize * p->B))) return bail(rc);
 	a.buf = p->buf;
 	a.n_ado = p->H;
 	a.M = p->M;
 	a.B = p->B;
 	a.logN = p->logN;
 	HX_HIP(hipMemsetAsync(p->rho, 0, p->elems * p->esize, p->stream));
-	if(p->rho0_f64) HX_HIP(hipMemsetAsync(p->rho0_f64, 0, static_cast<size_t>(p->N) * p->N * 16, p->stream));
+	if(p->rho0_f64) HX_HIP(hipMemsetAsync(p->rho0_f64, 0, p->pe * 16, p->stream));
 	HX_HIP(hipStreamSynchronize(p->stream));
 	*out = p;
 	return 0;
@@ -1311,7 +1521,7 @@ int helix_hip_set_initial(helix_hip_plan* p, const double* rho0)
 {
 	if(!p || !rho0) return fail(HELIX_HIP_EINVAL, "null argument");
 	const size_t n2 = static_cast<size_t>(p->N) * p->N;
-	std::vector<double> state(p->elems * 2, 0.0);
+	std::vector<double> state(static_cast<size_t>(p->H) * n2 * 2, 0.0);
 	std::memcpy(state.data(), rho0, n2 * 16);
 	return host_to_state(p, state.data());
 }
@@ -1359,10 +1569,12 @@ int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, do
 			const unsigned g = grid_for(n2);
 			double* ro = d_rho ? d_rho + r * n2 * 2 : nullptr;
 			double* dg = d_diag ? d_diag + r * static_cast<size_t>(p->N) : nullptr;
+			const int hm = p->herm ? 1 : 0;
 			if(p->precision == HELIX_HIP_FP64)
-				hipLaunchKernelGGL((k_record<double>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, nullptr, p->logN, ro, dg);
+				hipLaunchKernelGGL((k_record<double>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, nullptr, p->logN, hm, ro, dg);
 			else
-				hipLaunchKernelGGL((k_record<float>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, p->rho0_f64, p->logN, ro, dg);
+				hipLaunchKernelGGL((k_record<float>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, p->rho0_f64, p->logN, hm,
+					ro, dg);
 			++r;
 		}
 		if((s & 63) == 0)
@@ -1385,12 +1597,13 @@ int helix_hip_get_tier0(helix_hip_plan* p, double* out)
 {
 	if(!p || !out) return fail(HELIX_HIP_EINVAL, "null argument");
 	HX_HIP(hipStreamSynchronize(p->stream));
-	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	std::vector<double> st(p->pe * 2);
 	if(p->precision == HELIX_HIP_FP64) {
-		HX_HIP(hipMemcpy(out, p->rho, n2 * 16, hipMemcpyDeviceToHost));
+		HX_HIP(hipMemcpy(st.data(), p->rho, p->pe * 16, hipMemcpyDeviceToHost));
 	} else {
-		HX_HIP(hipMemcpy(out, p->rho0_f64, n2 * 16, hipMemcpyDeviceToHost));
+		HX_HIP(hipMemcpy(st.data(), p->rho0_f64, p->pe * 16, hipMemcpyDeviceToHost));
 	}
+	from_stored(st.data(), 1, p->logN, p->herm, out);
 	return 0;
 }
 
@@ -1442,6 +1655,7 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 		return 0;
 	}
 	if(!ados || count <= 0 || extra_dim < 0 || (extra_dim > 0 && !extra)) return fail(HELIX_HIP_EINVAL, "bad closure");
+	if(p->herm) return fail(HELIX_HIP_EUNSUPPORTED, "the learned closure needs a plan in HELIX_HIP_LAYOUT_FULL");
 	int din = 0, dout = 0;
 	helix_amd_mlp::dims(mlp, &din, &dout);
 	if(din != 2 * p->logN + extra_dim || dout != 2) return fail(HELIX_HIP_EINVAL, "MLP sizes must be (2 log2 N + extra, 2)");
@@ -1490,6 +1704,7 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 		return 0;
 	}
 	if(!ados || count <= 0) return fail(HELIX_HIP_EINVAL, "bad closure rates");
+	if(p->herm) return fail(HELIX_HIP_EUNSUPPORTED, "closure rates need a plan in HELIX_HIP_LAYOUT_FULL");
 	std::vector<int32_t> slot(p->H, -1);
 	for(int a = 0; a < count; ++a) {
 		if(ados[a] < 0 || ados[a] >= p->H) return fail(HELIX_HIP_EINVAL, "closure ADO index out of range");
@@ -1520,6 +1735,13 @@ int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* laun
 	if(launches_per_step) *launches_per_step = p->order * (p->flip ? 2 : 1);
 	if(uses_flip_coupling) *uses_flip_coupling = p->flip ? 1 : 0;
 	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = p->dense ? 1 : 0;
+	return 0;
+}
+
+int helix_hip_plan_layout(const helix_hip_plan* p, int32_t* layout)
+{
+	if(!p || !layout) return fail(HELIX_HIP_EINVAL, "null argument");
+	*layout = p->herm ? HELIX_HIP_LAYOUT_HERMITIAN : HELIX_HIP_LAYOUT_FULL;
 	return 0;
 }
 
@@ -1587,8 +1809,10 @@ struct helix_hip_batch {
 	int device = 0;
 	hipStream_t stream = nullptr;
 	int Q = 0, order = 4, precision = 0;
+	bool herm = false;  // every instance stored in the Hermitian layout
 	size_t esize = 16;
 	std::vector<int> N, logN, H;
+	std::vector<size_t> pe;  // stored elements per ADO of each instance
 	std::vector<size_t> elem_off, r0_off;
 	size_t total_elems = 0;
 	int64_t total_ado = 0;
@@ -1621,26 +1845,35 @@ int batch_alloc(helix_hip_batch* b, void** ptr, size_t bytes)
 	return 0;
 }
 
-template <typename T, bool SPLIT, int THREADS>
+template <typename T, bool SPLIT, int THREADS, bool HERM>
 void batch_launch_t(helix_hip_batch* b, int j)
 {
 	const unsigned grid = static_cast<unsigned>(((b->ntasks + kXcds - 1) / kXcds) * kXcds);
 	const StageArgs* tab = b->table + static_cast<size_t>(j - 1) * b->Q;
 	if(j == b->order)
-		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, true, THREADS>), dim3(grid), dim3(THREADS), 0, b->stream, tab, b->tasks,
-			b->ntasks);
-	else
-		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, false, THREADS>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, true, THREADS, HERM>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
 			b->tasks, b->ntasks);
+	else
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, false, THREADS, HERM>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
+			b->tasks, b->ntasks);
+}
+
+template <typename T, bool SPLIT, bool HERM>
+void batch_launch_h(helix_hip_batch* b, int j)
+{
+	if(b->threads == 1024)
+		batch_launch_t<T, SPLIT, 1024, HERM>(b, j);
+	else
+		batch_launch_t<T, SPLIT, 512, HERM>(b, j);
 }
 
 template <typename T, bool SPLIT>
 void batch_launch_p(helix_hip_batch* b, int j)
 {
-	if(b->threads == 1024)
-		batch_launch_t<T, SPLIT, 1024>(b, j);
+	if(b->herm)
+		batch_launch_h<T, SPLIT, true>(b, j);
 	else
-		batch_launch_t<T, SPLIT, 512>(b, j);
+		batch_launch_h<T, SPLIT, false>(b, j);
 }
 
 void batch_enqueue_step(helix_hip_batch* b)
@@ -1709,6 +1942,12 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 		if(pr->hamiltonian_dense) return fail(HELIX_HIP_EUNSUPPORTED, "batched form uses spin-structured H");
 		if(pr->n_modes > kTileMaxModes) return fail(HELIX_HIP_EUNSUPPORTED, "batched form supports up to 16 modes");
 	}
+	bool herm = true;
+	for(int q = 0; q < count; ++q)
+		if(problems[q]->layout == HELIX_HIP_LAYOUT_FULL || !preserves_hermiticity(problems[q])) herm = false;
+	for(int q = 0; q < count; ++q)
+		if(problems[q]->layout == HELIX_HIP_LAYOUT_HERMITIAN && !herm)
+			return fail(HELIX_HIP_EUNSUPPORTED, "Hermitian layout asked for a batch that cannot use it");
 	HX_HIP(hipSetDevice(device));
 	helix_hip_batch* b = new(std::nothrow) helix_hip_batch();
 	if(!b) return fail(HELIX_HIP_ENOMEM, "host allocation failed");
@@ -1721,6 +1960,7 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 	b->order = problems[0]->order;
 	b->precision = problems[0]->precision;
 	b->esize = (b->precision == HELIX_HIP_FP64) ? 16 : 8;
+	b->herm = herm;
 	if(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
 
@@ -1729,14 +1969,17 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 	for(int q = 0; q < count; ++q) {
 		const helix_hip_problem* pr = problems[q];
 		const int lg = ilog2_exact(pr->n);
+		const size_t n2 = static_cast<size_t>(pr->n) * pr->n;
+		const size_t pe = herm ? (n2 + pr->n) / 2 : n2;
 		b->N.push_back(pr->n);
 		b->logN.push_back(lg);
 		b->H.push_back(pr->n_ado);
+		b->pe.push_back(pe);
 		b->elem_off.push_back(b->total_elems);
-		b->total_elems += static_cast<size_t>(pr->n_ado) << (2 * lg);
+		b->total_elems += static_cast<size_t>(pr->n_ado) * pe;
 		b->total_ado += pr->n_ado;
 		b->r0_off.push_back(r0);
-		r0 += static_cast<size_t>(pr->n) * pr->n;
+		r0 += pe;
 	}
 	int rc = 0;
 	if((rc = batch_alloc(b, &b->rho, b->total_elems * b->esize))) return bail(rc);
@@ -1876,22 +2119,19 @@ int helix_hip_batch_set_initial(helix_hip_batch* b, int index, const double* rho
 	const int q0 = index < 0 ? 0 : index, q1 = index < 0 ? b->Q : index + 1;
 	for(int q = q0; q < q1; ++q) {
 		const size_t n2 = static_cast<size_t>(b->N[q]) * b->N[q];
-		const size_t elems = static_cast<size_t>(b->H[q]) * n2;
+		const size_t pe = b->pe[q];
 		std::vector<double> r(n2 * 2, 0.0);
 		if(index < 0)
 			r[2 * (n2 - 1)] = 1.0;  // |N-1><N-1|
 		else
 			std::memcpy(r.data(), rho0, n2 * 16);
+		std::vector<double> st;
+		if(int rc = to_stored(r.data(), 1, b->logN[q], b->herm, st)) return rc;
 		char* base = static_cast<char*>(b->rho) + b->elem_off[q] * b->esize;
-		HX_HIP(hipMemset(base, 0, elems * b->esize));
-		if(b->precision == HELIX_HIP_FP64) {
-			HX_HIP(hipMemcpy(base, r.data(), n2 * 16, hipMemcpyHostToDevice));
-		} else {
-			std::vector<float> f(n2 * 2);
-			for(size_t k = 0; k < n2 * 2; ++k) f[k] = static_cast<float>(r[k]);
-			HX_HIP(hipMemcpy(base, f.data(), n2 * 8, hipMemcpyHostToDevice));
-			HX_HIP(hipMemcpy(b->rho0_f64 + 2 * b->r0_off[q], r.data(), n2 * 16, hipMemcpyHostToDevice));
-		}
+		HX_HIP(hipMemset(base, 0, static_cast<size_t>(b->H[q]) * pe * b->esize));
+		if(int rc = upload_stored(base, st.data(), pe, b->precision == HELIX_HIP_FP64)) return rc;
+		if(b->precision != HELIX_HIP_FP64)
+			HX_HIP(hipMemcpy(b->rho0_f64 + 2 * b->r0_off[q], st.data(), pe * 16, hipMemcpyHostToDevice));
 	}
 	return 0;
 }
@@ -1914,11 +2154,13 @@ int helix_hip_batch_get_tier0(helix_hip_batch* b, int index, double* out)
 {
 	if(!b || !out || index < 0 || index >= b->Q) return fail(HELIX_HIP_EINVAL, "bad arguments");
 	HX_HIP(hipStreamSynchronize(b->stream));
-	const size_t n2 = static_cast<size_t>(b->N[index]) * b->N[index];
+	const size_t pe = b->pe[index];
+	std::vector<double> st(pe * 2);
 	if(b->precision == HELIX_HIP_FP64)
-		HX_HIP(hipMemcpy(out, static_cast<char*>(b->rho) + b->elem_off[index] * 16, n2 * 16, hipMemcpyDeviceToHost));
+		HX_HIP(hipMemcpy(st.data(), static_cast<char*>(b->rho) + b->elem_off[index] * 16, pe * 16, hipMemcpyDeviceToHost));
 	else
-		HX_HIP(hipMemcpy(out, b->rho0_f64 + 2 * b->r0_off[index], n2 * 16, hipMemcpyDeviceToHost));
+		HX_HIP(hipMemcpy(st.data(), b->rho0_f64 + 2 * b->r0_off[index], pe * 16, hipMemcpyDeviceToHost));
+	from_stored(st.data(), 1, b->logN[index], b->herm, out);
 	return 0;
 }
 
@@ -1926,17 +2168,14 @@ int helix_hip_batch_get_state(helix_hip_batch* b, int index, double* state)
 {
 	if(!b || !state || index < 0 || index >= b->Q) return fail(HELIX_HIP_EINVAL, "bad arguments");
 	HX_HIP(hipStreamSynchronize(b->stream));
-	const size_t n2 = static_cast<size_t>(b->N[index]) * b->N[index];
-	const size_t elems = static_cast<size_t>(b->H[index]) * n2;
+	const size_t pe = b->pe[index];
+	const size_t elems = static_cast<size_t>(b->H[index]) * pe;
 	const char* base = static_cast<const char*>(b->rho) + b->elem_off[index] * b->esize;
-	if(b->precision == HELIX_HIP_FP64) {
-		HX_HIP(hipMemcpy(state, base, elems * 16, hipMemcpyDeviceToHost));
-	} else {
-		std::vector<float> f(elems * 2);
-		HX_HIP(hipMemcpy(f.data(), base, elems * 8, hipMemcpyDeviceToHost));
-		for(size_t k = 0; k < elems * 2; ++k) state[k] = static_cast<double>(f[k]);
-		HX_HIP(hipMemcpy(state, b->rho0_f64 + 2 * b->r0_off[index], n2 * 16, hipMemcpyDeviceToHost));
-	}
+	std::vector<double> st(elems * 2);
+	if(int rc = download_stored(base, elems, b->precision == HELIX_HIP_FP64, st.data())) return rc;
+	if(b->precision != HELIX_HIP_FP64)
+		HX_HIP(hipMemcpy(st.data(), b->rho0_f64 + 2 * b->r0_off[index], pe * 16, hipMemcpyDeviceToHost));
+	from_stored(st.data(), b->H[index], b->logN[index], b->herm, state);
 	return 0;
 }
 
@@ -1953,6 +2192,13 @@ int helix_hip_batch_info(const helix_hip_batch* b, int64_t* total_ado, size_t* d
 	if(device_bytes) *device_bytes = b->device_bytes;
 	if(threads) *threads = b->threads;
 	if(graph) *graph = b->use_graph ? 1 : 0;
+	return 0;
+}
+
+int helix_hip_batch_layout(const helix_hip_batch* b, int32_t* layout)
+{
+	if(!b || !layout) return fail(HELIX_HIP_EINVAL, "null argument");
+	*layout = b->herm ? HELIX_HIP_LAYOUT_HERMITIAN : HELIX_HIP_LAYOUT_FULL;
 	return 0;
 }
 

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define HELIX_HIP_ABI_VERSION 1
+#define HELIX_HIP_ABI_VERSION 2
 
 /* contract error codes (< 0), cf. src/backend/transpose_backend.h:14-16 */
 #define HELIX_HIP_EINVAL (-1)
@@ -46,6 +46,21 @@ enum helix_hip_precision {
 	HELIX_HIP_FP64 = 0,       /* complex128 everywhere (the plain form) */
 	HELIX_HIP_FP32 = 1,       /* complex64 ADOs; tier-0 accumulation in FP64 */
 	HELIX_HIP_SPLIT_FP32 = 2  /* as FP32, with operator values split into two FP32 terms */
+};
+
+/* Storage of each ADO block in device memory.  When every exponent is real (Drude-Lorentz with Pade
+ * or Matsubara poles), down_comm is imaginary, down_anti real, the self and Xi coefficients real and
+ * KXiCounter imaginary, the Hamiltonian and couplings Hermitian, then L maps Hermitian ADOs to
+ * Hermitian ADOs and a Hermitian initial state keeps every ADO Hermitian.  The Hermitian layout
+ * stores N(N+1)/2 elements per ADO instead of N^2 (about half the HBM traffic):
+ *   main [N/2][N]: element (d, i) = rho[i][(i + d) mod N], 0 <= d < N/2
+ *   tail [N/2]:    element t      = rho[t][t + N/2]
+ * every other element is the conjugate of a stored one.  States passed in must then be Hermitian
+ * (to 1e-12 relative), else the call returns HELIX_HIP_EINVAL; states passed out are full blocks. */
+enum helix_hip_layout {
+	HELIX_HIP_LAYOUT_AUTO = 0,      /* Hermitian where the problem and the stage kernel allow it */
+	HELIX_HIP_LAYOUT_FULL = 1,      /* N^2 elements per ADO (any state; needed by the learned closure) */
+	HELIX_HIP_LAYOUT_HERMITIAN = 2  /* Hermitian layout or HELIX_HIP_EUNSUPPORTED */
 };
 
 /* A "spin-structured" operator: O = diag(d) + sum_f c_f X_f, (X_f)_{i, i^mask_f} = 1.
@@ -78,6 +93,7 @@ typedef struct helix_hip_problem {
 	int32_t precision;           /* enum helix_hip_precision */
 	int32_t order;               /* Taylor order m, 1 <= m <= 8 (ContextOptions::integrationOrder) */
 	double dt;                   /* time step (ContextOptions::timeStep) */
+	int32_t layout;              /* enum helix_hip_layout */
 } helix_hip_problem;
 
 typedef struct helix_hip_plan helix_hip_plan;
@@ -117,6 +133,9 @@ int helix_hip_time_steps(helix_hip_plan* plan, uint64_t nsteps, double* stage_ms
 int helix_hip_plan_info(const helix_hip_plan* plan, size_t* device_bytes, int* launches_per_step,
 	int* uses_flip_coupling, int* uses_dense_hamiltonian);
 
+/* The layout the plan stores its ADOs in (HELIX_HIP_LAYOUT_FULL or HELIX_HIP_LAYOUT_HERMITIAN). */
+int helix_hip_plan_layout(const helix_hip_plan* plan, int32_t* layout);
+
 void helix_hip_plan_destroy(helix_hip_plan* plan);
 
 /* Learned closure of the deepest kept tier (the network of include/helix_mlp.h).  No counterpart in the
@@ -126,7 +145,8 @@ void helix_hip_plan_destroy(helix_hip_plan* plan);
  *   x = [2 n_b(i) - 1 for b < log2 N, 2 n_b(j) - 1 for b < log2 N, extra[a][0 .. extra_dim)]
  * (the form of a Markovian terminator for sigma^z couplings).  The network runs once here, on the GPU,
  * and kappa stays resident; every stage kernel then applies it (linear, so exact in every Taylor stage).
- * mlp == NULL switches the closure off (plain truncation).  The MLP may be destroyed afterwards. */
+ * mlp == NULL switches the closure off (plain truncation).  The MLP may be destroyed afterwards.
+ * Needs a plan in HELIX_HIP_LAYOUT_FULL (the rates need not be conjugate-symmetric). */
 struct helix_hip_mlp;
 /* Same closure with the rates given directly: kappa [count][N][N][2] (e.g. an analytic terminator);
  * kappa == NULL switches it off. */
@@ -137,7 +157,8 @@ int helix_hip_plan_set_closure(helix_hip_plan* plan, struct helix_hip_mlp* mlp, 
 /* ---- batched form: many independent problems advanced by one launch per Taylor stage ----
  * No counterpart in the reference (one hard-coded problem per process, src/helix.cpp:73-104).
  * Every problem must have N <= 64, sigma^z couplings, a spin-structured H, and the same precision
- * and Taylor order (>= 2); dt and everything else may differ.  Flags: */
+ * and Taylor order (>= 2); dt and everything else may differ.  The batch stores every problem in the
+ * Hermitian layout when all of them allow it and none asks for HELIX_HIP_LAYOUT_FULL.  Flags: */
 #define HELIX_HIP_BATCH_GRAPH 1u     /* capture the m stage launches of a step in a hipGraph */
 #define HELIX_HIP_BATCH_AUTOTUNE 2u  /* time the workgroup shapes at creation and keep the fastest */
 
@@ -154,6 +175,7 @@ int helix_hip_batch_time_steps(helix_hip_batch* batch, uint64_t nsteps, double* 
 /* total ADOs, device bytes, workgroup threads chosen, graph in use */
 int helix_hip_batch_info(const helix_hip_batch* batch, int64_t* total_ado, size_t* device_bytes, int* threads,
 	int* graph);
+int helix_hip_batch_layout(const helix_hip_batch* batch, int32_t* layout);
 void helix_hip_batch_destroy(helix_hip_batch* batch);
 
 /* Message of the calling thread's last error ("" when none). */

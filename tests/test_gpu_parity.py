@@ -43,15 +43,16 @@ SPIN_CASES = [
 ]
 
 
-@pytest.mark.parametrize("kernel", ["tiled", "generic"])
+@pytest.mark.parametrize("kernel,layout", [("tiled", "hermitian"), ("tiled", "full"), ("generic", "full")])
 @pytest.mark.parametrize("case", SPIN_CASES, ids=lambda c: f"spins{c['spins']}_L{c['L']}")
-def test_rhs_parity_spin_glass_fp64(hx, oracle_mod, case, kernel, monkeypatch):
+def test_rhs_parity_spin_glass_fp64(hx, oracle_mod, case, kernel, layout, monkeypatch):
     if kernel == "generic":
         monkeypatch.setenv("HELIX_HIP_GENERIC_STAGE", "1")
     m = hx.Model.spin_glass(**case)
     bp = case.get("bath_params")
     p = oracle_mod.spin_glass_problem(case["spins"], case["seed"], L=case["L"], bath_params=bp)
-    plan = hx.Plan(m)
+    plan = hx.Plan(m, layout=layout)
+    assert plan.layout() == layout
     s = _rand_state(m.H, m.N, 11)
     plan.set_state(s)
     got = plan.rhs()

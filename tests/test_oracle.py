@@ -170,6 +170,27 @@ def test_rhs_trace_and_hermiticity(oracle_mod, which):
     assert np.max(np.abs(d[0] - d[0].conj().T)) <= 1e-12 * np.max(np.abs(d[0]))
 
 
+@pytest.mark.parametrize("which", ["legacy32", "tfim4_mixed_baths"])
+def test_every_ado_stays_hermitian(oracle_mod, which):
+    """The premise of the device's Hermitian storage layout: with real exponents L maps Hermitian ADOs to
+    Hermitian ADOs at every tier (down_comm imaginary, down_anti real, KXi real, KXiCounter imaginary), so
+    a Hermitian initial state keeps every ADO Hermitian."""
+    if which == "legacy32":
+        p = oracle_mod.legacy_problem(N=32)
+    else:
+        p = oracle_mod.spin_glass_problem(4, seed=2, L=3, bath_params=[(0.05, 0.5, 0.5), (0.4, 1.7, 1.5),
+                                                                       (0.2, 1.0, 0.6), (0.3, 0.6, 1.9)])
+    assert np.all(np.real(p.kxic) == 0) and np.all(np.imag(p.kxi) == 0)
+    rng = np.random.default_rng(3)
+    rho = rng.standard_normal((p.H, p.N, p.N)) + 1j * rng.standard_normal((p.H, p.N, p.N))
+    rho = rho + np.conj(np.transpose(rho, (0, 2, 1)))
+    d = p.rhs(rho)
+    assert np.max(np.abs(d - np.conj(np.transpose(d, (0, 2, 1))))) <= 1e-13 * np.max(np.abs(d))
+    r = p.taylor(p.initial(), 0.05, 40)
+    assert np.max(np.abs(r[1:])) > 1e-3  # the hierarchy is populated
+    assert np.max(np.abs(r - np.conj(np.transpose(r, (0, 2, 1))))) <= 1e-14
+
+
 def test_taylor4_equals_classical_rk4(oracle_mod):
     # src/liouville.cu:502-591: for a linear L the Taylor-4 recurrence is classical RK4
     p = oracle_mod.spin_glass_problem(3, seed=0, L=2)
