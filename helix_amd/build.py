@@ -21,7 +21,7 @@ CSRC = os.path.join(PKG, "csrc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["heom_plan.hip", "mlp.hip"]
-HOST_SOURCES = ["host/heom_setup.cpp", "host/setup_capi.cpp", "host/helix_api.cpp"]
+HOST_SOURCES = ["host/heom_setup.cpp", "host/setup_capi.cpp", "host/helix_api.cpp", "host/helix_io.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-shared",
@@ -62,12 +62,13 @@ def build(verbose: bool = False) -> dict:
           "-Wl,-rpath,$ORIGIN"])
     out["libhelix"] = host_lib
 
-    example = os.path.join(ROOT, "examples", "legacy_spin_glass.cpp")
-    if os.path.exists(example):
-        exe = os.path.join(PKG, "helix_example")
-        _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", example, "-o", exe, f"-L{PKG}", "-lhelix", "-lhelix_hip",
-              "-Wl,-rpath,$ORIGIN"])
-        out["helix_example"] = exe
+    for src, name in (("legacy_spin_glass.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint")):
+        example = os.path.join(ROOT, "examples", src)
+        if os.path.exists(example):
+            exe = os.path.join(PKG, name)
+            _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", example, "-o", exe, f"-L{PKG}", "-lhelix", "-lhelix_hip",
+                  "-Wl,-rpath,$ORIGIN"])
+            out[name] = exe
     if verbose:
         for k, v in out.items():
             print(f"built {k}: {v}")

@@ -38,12 +38,24 @@ SETUP_SYMBOLS = [
     "helix_setup_spin_glass_couplings", "helix_setup_hierarchy_size", "helix_setup_last_error",
 ]
 
+IO_SYMBOLS = [
+    "helix_io_write_energy_trace", "helix_io_hierarchy_hash", "helix_io_checkpoint_write", "helix_io_checkpoint_read",
+    "helix_io_last_error",
+]
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
 
 class SpinOperator(C.Structure):
     _fields_ = [("diag", _dp), ("nflips", C.c_int32), ("flip_masks", _ip), ("flip_coefs", _dp)]
+
+
+class CheckpointInfo(C.Structure):
+    """Mirror of helix_io_checkpoint."""
+    _fields_ = [("n", C.c_int32), ("n_ado", C.c_int32), ("n_modes", C.c_int32), ("n_baths", C.c_int32),
+                ("order", C.c_int32), ("precision", C.c_int32), ("steps", C.c_uint64), ("time", C.c_double),
+                ("dt", C.c_double), ("hierarchy_hash", C.c_uint64)]
 
 
 class Problem(C.Structure):
@@ -134,6 +146,12 @@ def host():
         L.helix_setup_hierarchy_size.argtypes = [C.c_int, C.c_int]
         L.helix_setup_hierarchy_size.restype = C.c_long
         L.helix_setup_last_error.restype = C.c_char_p
+        L.helix_io_write_energy_trace.argtypes = [C.c_char_p, _dp, _dp, C.c_int64, C.c_int]
+        L.helix_io_hierarchy_hash.argtypes = [_ip, C.c_int64, C.c_int32]
+        L.helix_io_hierarchy_hash.restype = C.c_uint64
+        L.helix_io_checkpoint_write.argtypes = [C.c_char_p, C.POINTER(CheckpointInfo), _dp]
+        L.helix_io_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(CheckpointInfo), _dp, C.c_uint64]
+        L.helix_io_last_error.restype = C.c_char_p
         _host = L
     return _host
 
@@ -275,6 +293,24 @@ class Plan:
         _check(hip().helix_hip_plan_layout(self._p, C.byref(v)), "helix_hip_plan_layout")
         return LAYOUT_NAMES[v.value]
 
+    def save_checkpoint(self, path: str, steps: int, time: float):
+        m = self.model
+        desc = m.problem_ptr().contents
+        info = CheckpointInfo(n=m.N, n_ado=m.H, n_modes=m.M, n_baths=desc.n_baths, order=desc.order,
+                              precision=desc.precision, steps=steps, time=time, dt=desc.dt,
+                              hierarchy_hash=hierarchy_hash(m.rows()))
+        checkpoint_write(path, info, self.get_state())
+
+    def load_checkpoint(self, path: str):
+        """Restores the state; returns (steps, time) stored with it."""
+        m = self.model
+        info, _ = checkpoint_read(path, with_state=False)
+        if (info.n, info.n_ado, info.n_modes, info.hierarchy_hash) != (m.N, m.H, m.M, hierarchy_hash(m.rows())):
+            raise HelixError("checkpoint belongs to another problem (Hilbert size or hierarchy differs)")
+        _, state = checkpoint_read(path)
+        self.set_state(state)
+        return info.steps, info.time
+
     def set_initial(self, rho0: np.ndarray | None = None):
         r = self.model.initial_rho() if rho0 is None else np.ascontiguousarray(rho0, np.complex128)
         _check(hip().helix_hip_set_initial(self._p, _pd(r.view(np.float64))), "helix_hip_set_initial")
@@ -403,3 +439,42 @@ def spin_glass_couplings(spins: int, seed: int) -> np.ndarray:
     J = np.zeros((spins, spins))
     _check(host().helix_setup_spin_glass_couplings(spins, seed, _pd(J)), "couplings", host())
     return J
+
+
+def _io_check(rc: int, what: str):
+    if rc != 0:
+        msg = host().helix_io_last_error()
+        raise HelixError(f"{what} failed: {msg.decode() if msg else ''}")
+
+
+def write_energy_trace(path: str, times, energies, float32: bool = True):
+    """The legacy CLI's outputEnergy.txt rows (include/helix_io.h)."""
+    t = np.ascontiguousarray(times, np.float64)
+    e = np.ascontiguousarray(energies, np.float64)
+    assert t.shape == e.shape
+    _io_check(host().helix_io_write_energy_trace(path.encode(), _pd(t), _pd(e), t.size, int(float32)),
+              "helix_io_write_energy_trace")
+
+
+def hierarchy_hash(rows: np.ndarray) -> int:
+    r = np.ascontiguousarray(rows, np.int32)
+    return int(host().helix_io_hierarchy_hash(r.ctypes.data_as(_ip), r.shape[0], r.shape[1]))
+
+
+def checkpoint_write(path: str, info: CheckpointInfo, state: np.ndarray):
+    s = np.ascontiguousarray(state, np.complex128)
+    assert s.shape == (info.n_ado, info.n, info.n)
+    _io_check(host().helix_io_checkpoint_write(path.encode(), C.byref(info), _pd(s.view(np.float64))),
+              "helix_io_checkpoint_write")
+
+
+def checkpoint_read(path: str, with_state: bool = True):
+    """(info, state or None) of a checkpoint file; raises on any header or hash mismatch."""
+    info = CheckpointInfo()
+    _io_check(host().helix_io_checkpoint_read(path.encode(), C.byref(info), None, 0), "helix_io_checkpoint_read")
+    if not with_state:
+        return info, None
+    s = np.zeros((info.n_ado, info.n, info.n), np.complex128)
+    _io_check(host().helix_io_checkpoint_read(path.encode(), C.byref(info), _pd(s.view(np.float64)), s.size * 2),
+              "helix_io_checkpoint_read")
+    return info, s

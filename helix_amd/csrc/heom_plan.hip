@@ -332,6 +332,7 @@ struct TileLds {
 	double* v;      // [16 N]   general diagonal couplings only
 	double2* selfx; // [N]      sigma^z couplings only
 	int* nb;        // [A][2 kTileMaxModes]
+	int* mb;        // [kTileMaxModes] bath of each mode
 };
 
 // Register slot r of a tile thread (lt = lane within its ADO): the stored element q within the ADO and
@@ -379,6 +380,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	double* __restrict__ s_v = lds.v;
 	double2* __restrict__ s_selfx = lds.selfx;
 	int (*s_nb)[2 * kTileMaxModes] = reinterpret_cast<int (*)[2 * kTileMaxModes]>(lds.nb);
+	int* __restrict__ s_mb = lds.mb;
 
 	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
@@ -397,6 +399,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	} else {
 		for(int q = tid; q < B * N; q += THREADS) s_v[q] = a.vdiag[q];
 	}
+	for(int q = tid; q < M; q += THREADS) s_mb[q] = a.mode_bath[q];
 	for(int q = tid; q < A * 2 * M; q += THREADS) {
 		const int s = q / (2 * M), k = q - s * (2 * M);
 		const int nn = (group / TPA) * A + s;
@@ -456,63 +459,77 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		ai[r] = -hr;
 	}
 
-	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}}
-	for(int q = 0; q < 2 * M; ++q) {
-		const int idx = s_nb[slot][q];
-		if(idx < 0) continue;
-		const bool down = q >= M;
-		const int k = down ? q - M : q;
-		const int b = a.mode_bath[k];
-		const V2* __restrict__ src = cur + static_cast<size_t>(idx) * PE;
-		V2 x[RT];
+	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}},
+	// walked NB entries at a time: every load of a batch (neighbour elements and coefficients) is issued
+	// before any arithmetic, so a thread waits for one memory round trip per NB neighbours -- small
+	// problems, with few workgroups in flight, are bound by that chain.
+	// (sized to what each variant's register budget holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
+	constexpr int NB = HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : (THREADS <= 512 ? 4 : 2))
+	                        : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
+	for(int q0 = 0; q0 < 2 * M; q0 += NB) {
+		int nid[NB];
+		double2 wc[NB], wa[NB];
+		V2 x[NB][RT];
 #pragma unroll
-		for(int r = 0; r < RT; ++r) {
-			x[r] = V2{T(0), T(0)};
-			if(!HX_SLOT_LIVE(r)) continue;
-			int qe, i, j;
-			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-			x[r] = src[qe];
+		for(int u = 0; u < NB; ++u) {
+			const int q = q0 + u;
+			nid[u] = (q < 2 * M) ? s_nb[slot][q] : -1;
+			wc[u] = make_double2(0.0, -1.0);  // up: weight -i on the commutator
+			wa[u] = make_double2(0.0, 0.0);
+#pragma unroll
+			for(int r = 0; r < RT; ++r) x[u][r] = V2{T(0), T(0)};
+			if(nid[u] < 0) continue;
+			if(q >= M) {
+				wc[u] = a.dcomm[n * M + (q - M)];
+				wa[u] = a.danti[n * M + (q - M)];
+			}
+			const V2* __restrict__ src = cur + static_cast<size_t>(nid[u]) * PE;
+#pragma unroll
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				int qe, i, j;
+				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+				x[u][r] = src[qe];
+			}
 		}
-		if constexpr(SZ) {
-			// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
-			// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
-			// Up:   (-i) d x.   Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
-			const int mask = a.sz_mask[b];
-			const double two_alpha = 2.0 * a.sz_alpha[b];
-			double wcomm = -1.0, wanti = 0.0;  // up: weight -i on the commutator part
-			if(down) {
-				wcomm = a.dcomm[n * M + k].y;
-				wanti = a.danti[n * M + k].x;
-			}
 #pragma unroll
-			for(int r = 0; r < RT; ++r) {
-				if(!HX_SLOT_LIVE(r)) continue;
-				int qe, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-				const int bi = i & mask, bj = j & mask;
-				const bool differ = bi != bj;
-				const double sg = bi ? two_alpha : -two_alpha;
-				const double w = sg * (differ ? wcomm : wanti);
-				// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
-				const T u = differ ? -x[r].y : x[r].x;
-				const T v = differ ? x[r].x : x[r].y;
-				rmadd<T, SPLIT>(ar[r], ai[r], w, u, v);
-			}
-		} else {
-			double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
-			if(down) {
-				wc = a.dcomm[n * M + k];
-				wa = a.danti[n * M + k];
-			}
-			const double* __restrict__ vb = s_v + b * N;
+		for(int u = 0; u < NB; ++u) {
+			if(nid[u] < 0) continue;
+			const int q = q0 + u;
+			const int b = s_mb[q >= M ? q - M : q];
+			if constexpr(SZ) {
+				// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
+				// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
+				// Up: (-i) d x.  Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
+				const int mask = a.sz_mask[b];
+				const double two_alpha = 2.0 * a.sz_alpha[b];
+				const double wcomm = wc[u].y, wanti = wa[u].x;
 #pragma unroll
-			for(int r = 0; r < RT; ++r) {
-				if(!HX_SLOT_LIVE(r)) continue;
-				int qe, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-				const double vi = vb[i], vj = vb[j];
-				const double d = vi - vj, s = vi + vj;
-				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
+				for(int r = 0; r < RT; ++r) {
+					if(!HX_SLOT_LIVE(r)) continue;
+					int qe, i, j;
+					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+					const int bi = i & mask, bj = j & mask;
+					const bool differ = bi != bj;
+					const double sg = bi ? two_alpha : -two_alpha;
+					const double w = sg * (differ ? wcomm : wanti);
+					// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
+					const T re = differ ? -x[u][r].y : x[u][r].x;
+					const T im = differ ? x[u][r].x : x[u][r].y;
+					rmadd<T, SPLIT>(ar[r], ai[r], w, re, im);
+				}
+			} else {
+				const double* __restrict__ vb = s_v + b * N;
+#pragma unroll
+				for(int r = 0; r < RT; ++r) {
+					if(!HX_SLOT_LIVE(r)) continue;
+					int qe, i, j;
+					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+					const double vi = vb[i], vj = vb[j];
+					const double d = vi - vj, sm = vi + vj;
+					cmadd<T, SPLIT>(ar[r], ai[r], wc[u].x * d + wa[u].x * sm, wc[u].y * d + wa[u].y * sm, x[u][r].x,
+						x[u][r].y);
+				}
 			}
 		}
 	}
@@ -595,8 +612,17 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 #undef HX_SLOT_LIVE
 }
 
+// waves per SIMD the tile kernel is compiled for: two workgroups per CU, one when the complex128 tile and the
+// N = 1024 self table (80 KiB of LDS) leave room for only one
+template <typename T, int LOGN, int THREADS>
+constexpr int tile_waves()
+{
+	return (sizeof(T) == 8 && LOGN >= 10) ? THREADS / 256 : THREADS / 128;
+}
+
 template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, LOGN, THREADS>(),
+	tile_waves<T, LOGN, THREADS>()))) void
 k_stage_tile(const StageArgs a, int ngroups)
 {
 	using V2 = typename C2t<T>::type;
@@ -607,11 +633,12 @@ k_stage_tile(const StageArgs a, int ngroups)
 	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
 	__shared__ double2 s_selfx[SZ ? N : 1];
 	__shared__ int s_nb[A * 2 * kTileMaxModes];
+	__shared__ int s_mb[kTileMaxModes];
 	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
 	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
 	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(group >= ngroups) return;
-	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb});
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_mb});
 }
 
 // Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
@@ -629,12 +656,13 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	__shared__ double s_v[1];
 	__shared__ double2 s_selfx[1 << kBatchMaxLogN];
 	__shared__ int s_nb[(kTile >> 6) * 2 * kTileMaxModes];
+	__shared__ int s_mb[kTileMaxModes];
 	const int per_xcd = (ntasks + kXcds - 1) / kXcds;
 	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
 	const StageArgs& a = table[tk.x];
-	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb};
+	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_mb};
 	switch(a.logN) {
 	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
 	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
@@ -957,6 +985,7 @@ struct helix_hip_plan {
 	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
 	bool dense = false;       // dense Hamiltonian on MFMA (N in {16, 32, 64})
 	bool herm = false;        // Hermitian layout: N(N+1)/2 stored elements per ADO
+	int threads = 0;          // tile workgroup size (512 or 1024; N <= 64 sigma^z tiles are autotuned)
 	size_t esize = 16;        // bytes per complex element
 	size_t pe = 0;            // stored elements per ADO (N^2, or N(N+1)/2 in the Hermitian layout)
 	size_t elems = 0;         // H * pe
@@ -968,6 +997,10 @@ struct helix_hip_plan {
 	size_t device_bytes = 0;
 	StageArgs base{};
 	std::vector<hipEvent_t> events;
+	// kChunkSteps steps captured once in a hipGraph and replayed by helix_hip_step (order >= 2); rebuilt
+	// after any change of the stage arguments
+	hipGraph_t chunk_graph = nullptr;
+	hipGraphExec_t chunk_exec = nullptr;
 	// learned closure of the deepest kept tier (helix_hip_plan_set_closure)
 	double2* kappa = nullptr;
 	int32_t* kslot = nullptr;
@@ -1085,11 +1118,23 @@ void launch_tile_v(helix_hip_plan* p, const StageArgs& a, bool last)
 			p->stream, a, ngroups);
 }
 
-// 512 threads for complex128 tiles (64 KiB LDS, 4 waves/SIMD), 1024 for complex64 tiles (32 KiB, 8 waves/SIMD)
+// 512 threads for complex128 tiles (64 KiB LDS, 4 waves/SIMD), 1024 for complex64 sigma^z tiles (32 KiB,
+// 8 waves/SIMD; general diagonal couplings need more registers and stay at 512); whole-block sigma^z tiles
+// (N <= 64) also have the other shape, which the plan's autotuner may pick
 template <typename T, bool SPLIT, int LOGN, bool SZ>
 void launch_tile_sz(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	constexpr int THREADS = (sizeof(T) == 8) ? 512 : 1024;
+	constexpr int THREADS = (sizeof(T) == 8 || !SZ) ? 512 : 1024;
+	constexpr int ALT = (sizeof(T) == 8) ? 1024 : 512;
+	if constexpr(LOGN <= 6 && SZ) {
+		if(p->threads == ALT) {
+			if(p->herm)
+				launch_tile_v<T, SPLIT, LOGN, ALT, SZ, true>(p, a, last);
+			else
+				launch_tile_v<T, SPLIT, LOGN, ALT, SZ, false>(p, a, last);
+			return;
+		}
+	}
 	if constexpr(LOGN <= 6) {
 		if(p->herm) {
 			launch_tile_v<T, SPLIT, LOGN, THREADS, SZ, true>(p, a, last);
@@ -1198,6 +1243,81 @@ void enqueue_stage(helix_hip_plan* p, int j)
 void enqueue_step(helix_hip_plan* p)
 {
 	for(int j = 1; j <= p->order; ++j) enqueue_stage(p, j);
+}
+
+// Steps replayed from one hipGraph: small problems are bound by launch latency (a 210-ADO N=8 stage
+// runs in a few microseconds), which the replay of a captured chunk removes.
+constexpr uint64_t kChunkSteps = 16;
+
+void drop_chunk_graph(helix_hip_plan* p)
+{
+	if(p->chunk_exec) (void)hipGraphExecDestroy(p->chunk_exec);
+	if(p->chunk_graph) (void)hipGraphDestroy(p->chunk_graph);
+	p->chunk_exec = nullptr;
+	p->chunk_graph = nullptr;
+}
+
+int enqueue_chunk(helix_hip_plan* p)
+{
+	if(!p->chunk_exec) {
+		HX_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+		for(uint64_t s = 0; s < kChunkSteps; ++s) enqueue_step(p);
+		hipError_t e = hipStreamEndCapture(p->stream, &p->chunk_graph);
+		if(e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+		e = hipGraphInstantiate(&p->chunk_exec, p->chunk_graph, nullptr, nullptr, 0);
+		if(e != hipSuccess) {
+			drop_chunk_graph(p);
+			return hip_fail(e, "hipGraphInstantiate");
+		}
+	}
+	HX_HIP(hipGraphLaunch(p->chunk_exec, p->stream));
+	return 0;
+}
+
+// Event time of `steps` eager steps on the plan's stream.
+int time_eager_steps(helix_hip_plan* p, int steps, double* ms)
+{
+	hipEvent_t e0, e1;
+	HX_HIP(hipEventCreate(&e0));
+	if(hipError_t e = hipEventCreate(&e1); e != hipSuccess) {
+		(void)hipEventDestroy(e0);
+		return hip_fail(e, "hipEventCreate");
+	}
+	hipError_t e = hipEventRecord(e0, p->stream);
+	for(int s = 0; s < steps && e == hipSuccess; ++s) enqueue_step(p);
+	if(e == hipSuccess) e = hipGetLastError();
+	if(e == hipSuccess) e = hipEventRecord(e1, p->stream);
+	if(e == hipSuccess) e = hipEventSynchronize(e1);
+	float f = 0.f;
+	if(e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if(e != hipSuccess) return hip_fail(e, "autotune timing");
+	*ms = f;
+	return 0;
+}
+
+// Whole-block sigma^z tiles: time both workgroup shapes on the all-zero state (stepping zero leaves it
+// zero, and the kernels move the same bytes whatever the values) and keep the faster.
+int autotune_threads(helix_hip_plan* p)
+{
+	const int def = (p->precision == HELIX_HIP_FP64) ? 512 : 1024;
+	p->threads = def;
+	if(!p->tiled || p->logN > 6 || !p->base.sz) return 0;
+	double best = 1e300;
+	int pick = def;
+	for(int th : {def, def == 512 ? 1024 : 512}) {
+		p->threads = th;
+		double ms = 0.0;
+		if(int rc = time_eager_steps(p, 1, &ms)) return rc;  // first launch of this shape
+		if(int rc = time_eager_steps(p, 3, &ms)) return rc;
+		if(ms < best) {
+			best = ms;
+			pick = th;
+		}
+	}
+	p->threads = pick;
+	return 0;
 }
 
 int launch_error(const char* what)
@@ -1503,6 +1623,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	HX_HIP(hipMemsetAsync(p->rho, 0, p->elems * p->esize, p->stream));
 	if(p->rho0_f64) HX_HIP(hipMemsetAsync(p->rho0_f64, 0, p->pe * 16, p->stream));
 	HX_HIP(hipStreamSynchronize(p->stream));
+	if((rc = autotune_threads(p))) return bail(rc);
 	*out = p;
 	return 0;
 }
@@ -1511,6 +1632,7 @@ void helix_hip_plan_destroy(helix_hip_plan* p)
 {
 	if(!p) return;
 	if(p->stream) (void)hipStreamSynchronize(p->stream);
+	drop_chunk_graph(p);
 	for(hipEvent_t e : p->events) (void)hipEventDestroy(e);
 	for(void* q : p->allocations) (void)hipFree(q);
 	if(p->stream) (void)hipStreamDestroy(p->stream);
@@ -1560,12 +1682,30 @@ int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, do
 	const size_t n2 = static_cast<size_t>(p->N) * p->N;
 	double* d_rho = nullptr;
 	double* d_diag = nullptr;
-	if(nrec && rho_trace) HX_HIP(hipMalloc(&d_rho, nrec * n2 * 16));
-	if(nrec && diag_trace) HX_HIP(hipMalloc(&d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double)));
+	auto done = [&](int rc) {
+		if(d_rho) (void)hipFree(d_rho);
+		if(d_diag) (void)hipFree(d_diag);
+		return rc;
+	};
+	if(nrec && rho_trace && hipMalloc(&d_rho, nrec * n2 * 16) != hipSuccess)
+		return done(fail(HELIX_HIP_ENOMEM, "trace buffer"));
+	if(nrec && diag_trace && hipMalloc(&d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double)) != hipSuccess)
+		return done(fail(HELIX_HIP_ENOMEM, "trace buffer"));
 	uint64_t r = 0;
-	for(uint64_t s = 1; s <= nsteps; ++s) {
-		enqueue_step(p);
-		if(nrec && s % record_every == 0 && r < nrec) {
+	uint64_t s = 0;
+	while(s < nsteps) {
+		// steps up to the next record point (or the end), in graph chunks where they fit
+		const uint64_t next = (nrec && r < nrec) ? (r + 1) * record_every : nsteps;
+		while(s < next) {
+			if(p->order >= 2 && next - s >= kChunkSteps) {
+				if(int rc = enqueue_chunk(p)) return done(rc);
+				s += kChunkSteps;
+			} else {
+				enqueue_step(p);
+				++s;
+			}
+		}
+		if(nrec && r < nrec && s == (r + 1) * record_every) {
 			const unsigned g = grid_for(n2);
 			double* ro = d_rho ? d_rho + r * n2 * 2 : nullptr;
 			double* dg = d_diag ? d_diag + r * static_cast<size_t>(p->N) : nullptr;
@@ -1577,20 +1717,16 @@ int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, do
 					ro, dg);
 			++r;
 		}
-		if((s & 63) == 0)
-			if(int rc = launch_error("step launch")) return rc;
+		if(int rc = launch_error("step launch")) return done(rc);
 	}
-	if(int rc = launch_error("step launch")) return rc;
-	HX_HIP(hipStreamSynchronize(p->stream));
-	if(d_rho) {
-		HX_HIP(hipMemcpy(rho_trace, d_rho, nrec * n2 * 16, hipMemcpyDeviceToHost));
-		(void)hipFree(d_rho);
-	}
-	if(d_diag) {
-		HX_HIP(hipMemcpy(diag_trace, d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double), hipMemcpyDeviceToHost));
-		(void)hipFree(d_diag);
-	}
-	return 0;
+	if(int rc = launch_error("step launch")) return done(rc);
+	if(hipError_t e = hipStreamSynchronize(p->stream); e != hipSuccess) return done(hip_fail(e, "step"));
+	if(d_rho && hipMemcpy(rho_trace, d_rho, nrec * n2 * 16, hipMemcpyDeviceToHost) != hipSuccess)
+		return done(fail(HELIX_HIP_ESTATE, "trace copy failed"));
+	if(d_diag && hipMemcpy(diag_trace, d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double), hipMemcpyDeviceToHost) !=
+			hipSuccess)
+		return done(fail(HELIX_HIP_ESTATE, "trace copy failed"));
+	return done(0);
 }
 
 int helix_hip_get_tier0(helix_hip_plan* p, double* out)
@@ -1649,6 +1785,7 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 {
 	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
 	HX_HIP(hipStreamSynchronize(p->stream));
+	drop_chunk_graph(p);
 	if(!mlp) {
 		p->base.kappa = nullptr;
 		p->base.kslot = nullptr;
@@ -1698,6 +1835,7 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 {
 	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
 	HX_HIP(hipStreamSynchronize(p->stream));
+	drop_chunk_graph(p);
 	if(!kappa) {
 		p->base.kappa = nullptr;
 		p->base.kslot = nullptr;

@@ -8,6 +8,7 @@
 
 #include "heom_setup.hpp"
 #include "helix_hip.h"
+#include "helix_io.h"
 
 #include <algorithm>
 #include <atomic>
@@ -423,7 +424,40 @@ public:
 	{
 		require_active();
 		check(helix_hip_step(plan_, steps, 0, nullptr, nullptr), "helix_hip_step");
-		steps_ += steps;
+		advance_clock(steps);
+	}
+
+	std::size_t steps_taken() const noexcept { return steps_; }
+	double elapsed_time() const noexcept { return time_; }
+
+	void save_checkpoint(const std::string& path) const
+	{
+		require_active();
+		const std::size_t n2 = static_cast<std::size_t>(problem_->n()) * problem_->n();
+		std::vector<double> state(static_cast<std::size_t>(problem_->ado_count()) * n2 * 2);
+		check(helix_hip_get_state(plan_, state.data()), "helix_hip_get_state");
+		const helix_io_checkpoint info = checkpoint_info();
+		if(helix_io_checkpoint_write(path.c_str(), &info, state.data()) != 0)
+			throw std::runtime_error(std::string("save_checkpoint: ") + helix_io_last_error());
+	}
+
+	void load_checkpoint(const std::string& path)
+	{
+		require_active();
+		helix_io_checkpoint info{};
+		if(helix_io_checkpoint_read(path.c_str(), &info, nullptr, 0) != 0)
+			throw std::runtime_error(std::string("load_checkpoint: ") + helix_io_last_error());
+		const helix_io_checkpoint mine = checkpoint_info();
+		if(info.n != mine.n || info.n_ado != mine.n_ado || info.n_modes != mine.n_modes ||
+			info.hierarchy_hash != mine.hierarchy_hash)
+			throw std::invalid_argument("load_checkpoint: the checkpoint belongs to another problem (Hilbert size or "
+										"hierarchy differs)");
+		std::vector<double> state(static_cast<std::size_t>(info.n_ado) * info.n * info.n * 2);
+		if(helix_io_checkpoint_read(path.c_str(), &info, state.data(), state.size()) != 0)
+			throw std::runtime_error(std::string("load_checkpoint: ") + helix_io_last_error());
+		check(helix_hip_set_state(plan_, state.data()), "helix_hip_set_state");
+		steps_ = static_cast<std::size_t>(info.steps);
+		time_ = info.time;
 	}
 
 	std::vector<std::complex<double>> reduced_density() const
@@ -444,7 +478,7 @@ public:
 		check(helix_hip_get_tier0(plan_, reinterpret_cast<double*>(rho.data())), "helix_hip_get_tier0");
 		if(steps)
 			check(helix_hip_step(plan_, steps, 1, reinterpret_cast<double*>(rho.data() + n2), nullptr), "helix_hip_step");
-		steps_ += steps;
+		advance_clock(steps);
 		ediag.resize(steps + 1);
 		etr.resize(steps + 1);
 		for(std::size_t s = 0; s <= steps; ++s) {
@@ -485,6 +519,28 @@ private:
 	{
 		if(!plan_) throw std::logic_error("the Context was destroyed");
 	}
+	// step count and time, the time summed step by step (the legacy CLI's `time += Param::Step`)
+	void advance_clock(std::size_t steps)
+	{
+		for(std::size_t s = 0; s < steps; ++s) time_ += options_.timeStep;
+		steps_ += steps;
+	}
+	helix_io_checkpoint checkpoint_info() const
+	{
+		helix_io_checkpoint info{};
+		info.n = problem_->n();
+		info.n_ado = static_cast<int32_t>(problem_->ado_count());
+		info.n_modes = problem_->modes();
+		info.n_baths = static_cast<int32_t>(problem_->baths().size());
+		info.order = options_.integrationOrder;
+		info.precision = static_cast<int32_t>(options_.precision);
+		info.steps = steps_;
+		info.time = time_;
+		info.dt = options_.timeStep;
+		const auto& rows = problem_->hierarchy().rows;
+		info.hierarchy_hash = helix_io_hierarchy_hash(rows.data(), problem_->ado_count(), problem_->modes());
+		return info;
+	}
 
 	ContextOptions options_;
 	Diagnostics diagnostics_;
@@ -492,6 +548,7 @@ private:
 	std::unique_ptr<setup::Problem> problem_;
 	helix_hip_plan* plan_ = nullptr;
 	std::size_t steps_ = 0;
+	double time_ = 0.0;
 };
 
 Context::Context(ContextOptions options) : impl_(std::make_unique<Impl>(options, nullptr, nullptr)) {}
@@ -572,6 +629,28 @@ void Context::run_steps_observed(std::size_t steps, std::vector<std::complex<dou
 	impl_->run_steps_observed(steps, rho_trace, energy_diagonal_trace, energy_trace);
 }
 
+std::size_t Context::steps_taken() const noexcept
+{
+	return impl_ ? impl_->steps_taken() : 0;
+}
+
+double Context::elapsed_time() const noexcept
+{
+	return impl_ ? impl_->elapsed_time() : 0.0;
+}
+
+void Context::save_checkpoint(const std::string& path) const
+{
+	if(!impl_) throw std::logic_error("moved-from Context");
+	impl_->save_checkpoint(path);
+}
+
+void Context::load_checkpoint(const std::string& path)
+{
+	if(!impl_) throw std::logic_error("moved-from Context");
+	impl_->load_checkpoint(path);
+}
+
 // ------------------------------------------------------------ HEOMSolver
 
 HEOMSolver::HEOMSolver() = default;
@@ -607,8 +686,10 @@ RunResult HEOMSolver::run_steps(std::size_t steps)
 		const std::size_t N = context_->hilbert_size();
 		const double dt = context_->time_step();
 		if(solverOptions_.resultMode == ResultMode::ObservableTrace) {
+			// times as the legacy CLI prints them: summed step by step from the context's clock
+			double t = context_->elapsed_time();
 			context_->run_steps_observed(steps, result.reduced_density, result.energy_diagonal, result.energy);
-			for(std::size_t s = 0; s <= steps; ++s) result.times.push_back(static_cast<double>(s) * dt);
+			for(std::size_t s = 0; s <= steps; ++s, t += dt) result.times.push_back(t);
 			result.reduced_density_shape = {steps + 1, N, N, MatrixStorageOrder::RowMajor};
 		} else {
 			context_->run_steps(steps);

@@ -216,6 +216,12 @@ public:
 	double energy() const;
 	void run_steps_observed(std::size_t steps, std::vector<std::complex<double>>& rho_trace,
 		std::vector<double>& energy_diagonal_trace, std::vector<double>& energy_trace);
+	std::size_t steps_taken() const noexcept;
+	double elapsed_time() const noexcept;  // the time steps taken, summed one by one as the legacy CLI does
+	// FP64 checkpoint of the whole hierarchy (format in helix_io.h); load_checkpoint needs a context of the
+	// same problem (Hilbert size, hierarchy and its hash) and restores the step count and time.
+	void save_checkpoint(const std::string& path) const;
+	void load_checkpoint(const std::string& path);
 
 private:
 	class Impl;

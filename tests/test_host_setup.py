@@ -25,7 +25,7 @@ def hx():
 def _declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(helix_(?:hip|model|setup)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(helix_(?:hip|model|setup|io)_\w+)\s*\(", text)))
 
 
 def test_libraries_export_every_declared_symbol(hx):
@@ -39,6 +39,9 @@ def test_libraries_export_every_declared_symbol(hx):
         assert hasattr(hip, name), name
     assert set(_declared("helix_hip.h")) == set(capi.HIP_SYMBOLS)
     assert set(_declared("helix_setup.h")) == set(capi.SETUP_SYMBOLS)
+    for name in _declared("helix_io.h"):
+        assert hasattr(host, name), name
+    assert set(_declared("helix_io.h")) == set(capi.IO_SYMBOLS)
     assert len(_declared("helix_mlp.h")) == 9
 
 
