@@ -62,7 +62,8 @@ def build(verbose: bool = False) -> dict:
           "-Wl,-rpath,$ORIGIN"])
     out["libhelix"] = host_lib
 
-    for src, name in (("legacy_spin_glass.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint")):
+    for src, name in (("legacy_spin_glass.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint"),
+                      ("sparse_system.cpp", "helix_sparse_system")):
         example = os.path.join(ROOT, "examples", src)
         if os.path.exists(example):
             exe = os.path.join(PKG, name)

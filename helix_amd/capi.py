@@ -31,7 +31,8 @@ HIP_SYMBOLS = [
     "helix_hip_plan_layout", "helix_hip_batch_layout",
 ]
 SETUP_SYMBOLS = [
-    "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_use_dense_hamiltonian",
+    "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
+    "helix_model_use_dense_hamiltonian",
     "helix_model_problem",
     "helix_model_rows",
     "helix_model_energy_diagonal", "helix_model_energy_trace", "helix_model_free", "helix_setup_pade",
@@ -66,6 +67,7 @@ class Problem(C.Structure):
         ("down_comm", _dp), ("down_anti", _dp), ("self_coef", _dp), ("xi", _dp), ("xi_counter", _dp),
         ("hamiltonian", SpinOperator), ("hamiltonian_dense", _dp), ("couplings", C.POINTER(SpinOperator)),
         ("precision", C.c_int32), ("order", C.c_int32), ("dt", C.c_double), ("layout", C.c_int32),
+        ("hamiltonian_csr", C.c_void_p), ("couplings_csr", C.c_void_p),
     ]
 
 
@@ -130,6 +132,8 @@ def host():
         L.helix_model_spin_glass.argtypes = [C.c_int, C.c_uint64, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int,
                                              C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
         L.helix_model_use_dense_hamiltonian.argtypes = [vp, C.c_int]
+        L.helix_model_sparse.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp,
+                                         C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
         L.helix_model_spin_glass_random.argtypes = [C.c_uint64, C.c_int, C.c_double, C.c_int, C.POINTER(vp),
                                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.helix_model_problem.argtypes = [vp]
@@ -200,6 +204,39 @@ class Model:
                                            PRECISIONS[precision], C.byref(hdl))
         _check(rc, "helix_model_spin_glass", host())
         return cls(hdl, 1 << spins)
+
+    @classmethod
+    def sparse(cls, hamiltonian, couplings, baths, K=1, max_tier=2, order=4, dt=0.01, precision="fp64"):
+        """A general sparse system (System::from_sparse): Hermitian H and couplings given as dense arrays
+        (their nonzeros are taken), one bath (beta, gamma, zeta) per coupling -- the reference's Bath
+        fields, lambda = zeta gamma / 2.  Runs on the generic CSR kernels."""
+        def csr(a):
+            a = np.asarray(a, np.complex128)
+            off, col, val = [0], [], []
+            for i in range(a.shape[0]):
+                nz = np.nonzero(a[i])[0]
+                col.extend(nz.tolist())
+                val.extend(a[i, nz].tolist())
+                off.append(len(col))
+            return (np.array(off, np.int32), np.array(col, np.int32),
+                    np.ascontiguousarray(np.array(val, np.complex128)).view(np.float64))
+        N = np.asarray(hamiltonian).shape[0]
+        ho, hc, hv = csr(hamiltonian)
+        parts = [csr(v) for v in couplings]
+        v_nnz = np.array([len(c) for _, c, _ in parts], np.int32)
+        v_off = np.ascontiguousarray(np.concatenate([o for o, _, _ in parts]), np.int32)
+        v_col = np.ascontiguousarray(np.concatenate([c for _, c, _ in parts]), np.int32)
+        v_val = np.ascontiguousarray(np.concatenate([v for _, _, v in parts]), np.float64)
+        beta = np.array([b[0] for b in baths], np.float64)
+        gamma = np.array([b[1] for b in baths], np.float64)
+        zeta = np.array([b[2] for b in baths], np.float64)
+        assert len(baths) == len(couplings)
+        hdl = C.c_void_p()
+        ip = lambda a: a.ctypes.data_as(_ip)  # noqa: E731
+        _check(host().helix_model_sparse(N, len(hc), ip(ho), ip(hc), _pd(hv), len(couplings), ip(v_nnz), ip(v_off),
+                                         ip(v_col), _pd(v_val), _pd(beta), _pd(gamma), _pd(zeta), K, max_tier, order,
+                                         dt, PRECISIONS[precision], C.byref(hdl)), "helix_model_sparse", host())
+        return cls(hdl, N)
 
     @classmethod
     def spin_glass_random(cls, seed, order=4, dt=0.01, precision="fp64"):

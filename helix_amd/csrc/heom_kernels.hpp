@@ -63,6 +63,14 @@ struct StageArgs {
 	// learned closure of the deepest kept tier: L(rho)_n += kappa[kslot[n]][e] rho_n[e] where kslot[n] >= 0
 	const double2* kappa;     // [count][N^2] or null
 	const int32_t* kslot;     // [H]
+	// general sparse Hermitian operators (CSR): H, and every V_b with its row offsets [B][N+1] absolute
+	// into the concatenated column/value arrays
+	const int32_t* hcsr_off;  // [N+1]
+	const int32_t* hcsr_col;
+	const double2* hcsr_val;
+	const int32_t* vcsr_off;  // [B][N+1]
+	const int32_t* vcsr_col;
+	const double2* vcsr_val;
 };
 
 }  // namespace helix_amd

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <new>
 #include <string>
 #include <vector>
@@ -119,8 +120,48 @@ __device__ __forceinline__ void flip_comm_anti(const typename C2t<T>::type* __re
 	}
 }
 
-// Pass A of a stage with bit-flip couplings: buf[b][n] = [V_b, cur_n]  (src/liouville.cu:792-801).
-template <typename T, bool SPLIT>
+// [O,X]_ij and {O,X}_ij for a general sparse Hermitian O in CSR form (row offsets `off`):
+//   (O X)_ij = sum_{p in row i} o_p X[col_p][j],   (X O)_ij = sum_k X_ik O_kj = sum_{p in row j} X[i][col_p] conj(o_p)
+// (Hermiticity turns the column walk of O into a row walk).  The reference computes the same with two
+// column-major cuSPARSE SpMMs and transposes (src/liouville.cu:275-366).
+template <typename T, bool SPLIT, bool ANTI>
+__device__ __forceinline__ void csr_comm_anti(const typename C2t<T>::type* __restrict__ X, const int32_t* __restrict__ off,
+	const int32_t* __restrict__ col, const double2* __restrict__ val, int logN, uint32_t i, uint32_t j, T& cr, T& ci,
+	T& sr, T& si)
+{
+	cr = ci = sr = si = T(0);
+	for(int p = off[i]; p < off[i + 1]; ++p) {
+		const double2 o = val[p];
+		const auto x = X[(static_cast<uint32_t>(col[p]) << logN) | j];
+		cmadd<T, SPLIT>(cr, ci, o.x, o.y, x.x, x.y);
+		if constexpr(ANTI) cmadd<T, SPLIT>(sr, si, o.x, o.y, x.x, x.y);
+	}
+	for(int p = off[j]; p < off[j + 1]; ++p) {
+		const double2 o = val[p];  // X O picks conj(o): subtract it from the commutator, add it to the anticommutator
+		const auto x = X[(i << logN) | static_cast<uint32_t>(col[p])];
+		cmadd<T, SPLIT>(cr, ci, -o.x, o.y, x.x, x.y);
+		if constexpr(ANTI) cmadd<T, SPLIT>(sr, si, o.x, -o.y, x.x, x.y);
+	}
+}
+
+// coupling operator kinds of the generic stage kernels
+constexpr int kVDiag = 0;  // every V_b diagonal: all V terms elementwise
+constexpr int kVFlip = 1;  // V_b = diag + bit flips
+constexpr int kVCsr = 2;   // V_b general sparse (CSR)
+
+template <typename T, bool SPLIT, bool ANTI, int VOP>
+__device__ __forceinline__ void coupling_comm_anti(const typename C2t<T>::type* __restrict__ X, const StageArgs& a, int b,
+	uint32_t i, uint32_t j, uint32_t e, T& cr, T& ci, T& sr, T& si)
+{
+	if constexpr(VOP == kVCsr)
+		csr_comm_anti<T, SPLIT, ANTI>(X, a.vcsr_off + b * ((1 << a.logN) + 1), a.vcsr_col, a.vcsr_val, a.logN, i, j, cr, ci,
+			sr, si);
+	else
+		flip_comm_anti<T, SPLIT, ANTI>(X, a, b, i, j, e, cr, ci, sr, si);
+}
+
+// Pass A of a stage with non-diagonal couplings: buf[b][n] = [V_b, cur_n]  (src/liouville.cu:792-801).
+template <typename T, bool SPLIT, int VOP>
 __global__ __launch_bounds__(kBlock) void k_flip_commutator(const StageArgs a)
 {
 	using V2 = typename C2t<T>::type;
@@ -136,18 +177,21 @@ __global__ __launch_bounds__(kBlock) void k_flip_commutator(const StageArgs a)
 		const int b = static_cast<int>(bn / a.n_ado);
 		const uint32_t n = bn - static_cast<uint32_t>(b) * a.n_ado;
 		T cr, ci, sr, si;
-		flip_comm_anti<T, SPLIT, false>(cur + (static_cast<size_t>(n) << logE), a, b, e >> a.logN, e & (N - 1), e, cr,
-			ci, sr, si);
+		coupling_comm_anti<T, SPLIT, false, VOP>(cur + (static_cast<size_t>(n) << logE), a, b, e >> a.logN, e & (N - 1), e,
+			cr, ci, sr, si);
 		buf[g] = V2{cr, ci};
 	}
 }
 
 // One Taylor stage: L(cur) for every ADO element with the recurrence epilogue.
-// FLIP=false: every V_b diagonal, so [V_b, X] and {V_b, X} are elementwise and the whole RHS is one pass.
-// FLIP=true : V_b with bit flips; [V_b, cur] comes from pass A (buf), {V_b, .} terms are gathered here.
-template <typename T, bool SPLIT, bool LAST, bool FLIP>
+// VOP = kVDiag: every V_b diagonal, so [V_b, X] and {V_b, X} are elementwise and the whole RHS is one pass.
+// otherwise   : V_b with bit flips or general sparse; [V_b, cur] comes from pass A (buf), {V_b, .} terms
+//               are gathered here.
+// HCSR: H is a general sparse operator (CSR) instead of diag + bit flips.
+template <typename T, bool SPLIT, bool LAST, int VOP, bool HCSR>
 __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 {
+	constexpr bool FLIP = VOP != kVDiag;
 	using V2 = typename C2t<T>::type;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
 	const V2* __restrict__ buf = static_cast<const V2*>(a.buf);
@@ -165,15 +209,20 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 		const V2 c = cn[e];
 		T ar = T(0), ai = T(0);
 
-		// -i[H, rho_n] (diagonal part as in src/liouville.cu:368-386, plus bit-flip terms)
+		// -i[H, rho_n] (diagonal part as in src/liouville.cu:368-386, plus bit-flip terms, or a general CSR H)
 		{
 			T hr = T(0), hi = T(0);
-			cmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], 0.0, c.x, c.y);
-			for(int f = 0; f < a.hflips; ++f) {
-				const uint32_t m = static_cast<uint32_t>(a.hmask[f]);
-				const V2 x1 = cn[e ^ (m << logN)];
-				const V2 x2 = cn[e ^ m];
-				cmadd<T, SPLIT>(hr, hi, a.hcoef[f], 0.0, x1.x - x2.x, x1.y - x2.y);
+			if constexpr(HCSR) {
+				T sr, si;
+				csr_comm_anti<T, SPLIT, false>(cn, a.hcsr_off, a.hcsr_col, a.hcsr_val, logN, i, j, hr, hi, sr, si);
+			} else {
+				cmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], 0.0, c.x, c.y);
+				for(int f = 0; f < a.hflips; ++f) {
+					const uint32_t m = static_cast<uint32_t>(a.hmask[f]);
+					const V2 x1 = cn[e ^ (m << logN)];
+					const V2 x2 = cn[e ^ m];
+					cmadd<T, SPLIT>(hr, hi, a.hcoef[f], 0.0, x1.x - x2.x, x1.y - x2.y);
+				}
 			}
 			ar += hi;
 			ai -= hr;
@@ -215,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 				cmadd<T, SPLIT>(ar, ai, wc.x, wc.y, x.x, x.y);
 				if(wa.x != 0.0 || wa.y != 0.0) {
 					T cr, ci, sr, si;
-					flip_comm_anti<T, SPLIT, true>(cur + (static_cast<size_t>(dn) << logE), a, b, i, j, e, cr, ci, sr, si);
+					coupling_comm_anti<T, SPLIT, true, VOP>(cur + (static_cast<size_t>(dn) << logE), a, b, i, j, e, cr, ci,
+						sr, si);
 					cmadd<T, SPLIT>(ar, ai, wa.x, wa.y, sr, si);
 				}
 			}
@@ -245,8 +295,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 				cmadd<T, SPLIT>(ar, ai, sc.x, sc.y, c.x, c.y);
 				for(int b = 0; b < a.B; ++b) {
 					T cr, ci, sr, si;
-					flip_comm_anti<T, SPLIT, true>(buf + (static_cast<size_t>(b * a.n_ado + n) << logE), a, b, i, j, e, cr,
-						ci, sr, si);
+					coupling_comm_anti<T, SPLIT, true, VOP>(buf + (static_cast<size_t>(b * a.n_ado + n) << logE), a, b, i, j,
+						e, cr, ci, sr, si);
 					cmadd<T, SPLIT>(ar, ai, -a.xi[b].x, -a.xi[b].y, cr, ci);
 					cmadd<T, SPLIT>(ar, ai, -a.xic[b].x, -a.xic[b].y, sr, si);
 				}
@@ -981,7 +1031,9 @@ struct helix_hip_plan {
 	hipStream_t stream = nullptr;
 	int N = 0, logN = 0, H = 0, M = 0, B = 0, order = 4, precision = 0;
 	double dt = 0.0;
-	bool flip = false;
+	bool flip = false;        // some coupling has bit flips
+	bool csr_h = false;       // general sparse H (CSR)
+	bool csr_v = false;       // general sparse couplings (CSR)
 	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
 	bool dense = false;       // dense Hamiltonian on MFMA (N in {16, 32, 64})
 	bool herm = false;        // Hermitian layout: N(N+1)/2 stored elements per ADO
@@ -1037,11 +1089,64 @@ int upload(helix_hip_plan* p, const X* host, size_t count, const X** dev)
 	return 0;
 }
 
+// count CSR operators (each [n+1] offsets) concatenated: offsets made absolute into one column / value array
+int upload_csr(helix_hip_plan* p, const helix_hip_csr_operator* ops, int count, int n, const int32_t** off,
+	const int32_t** col, const double2** val)
+{
+	std::vector<int32_t> o, c;
+	std::vector<double> v;
+	for(int q = 0; q < count; ++q) {
+		const int32_t base = static_cast<int32_t>(c.size());
+		for(int i = 0; i <= n; ++i) o.push_back(base + ops[q].row_offsets[i]);
+		c.insert(c.end(), ops[q].columns, ops[q].columns + ops[q].nnz);
+		v.insert(v.end(), ops[q].values, ops[q].values + 2 * static_cast<size_t>(ops[q].nnz));
+	}
+	if(c.empty()) {  // keep valid pointers for an all-zero operator
+		c.push_back(0);
+		v.push_back(0.0);
+		v.push_back(0.0);
+	}
+	int rc = upload(p, o.data(), o.size(), off);
+	if(!rc) rc = upload(p, c.data(), c.size(), col);
+	if(!rc) rc = upload(p, reinterpret_cast<const double2*>(v.data()), v.size() / 2, val);
+	return rc;
+}
+
 int ilog2_exact(int v)
 {
 	int l = 0;
 	while((1 << l) < v) ++l;
 	return ((1 << l) == v) ? l : -1;
+}
+
+// A CSR operator: well-formed offsets and columns, and Hermitian (O_ji = conj O_ij to 1e-12 of max|O|,
+// duplicates summed) -- the X O product of the commutators walks rows of O.
+int check_csr(const helix_hip_csr_operator& op, int n, const std::string& name)
+{
+	if(!op.row_offsets || (op.nnz > 0 && (!op.columns || !op.values)) || op.nnz < 0)
+		return fail(HELIX_HIP_EINVAL, name + ": null arrays");
+	if(op.row_offsets[0] != 0 || op.row_offsets[n] != op.nnz) return fail(HELIX_HIP_EINVAL, name + ": bad row offsets");
+	std::map<std::pair<int, int>, std::pair<double, double>> entries;
+	double mx = 0.0;
+	for(int i = 0; i < n; ++i) {
+		if(op.row_offsets[i + 1] < op.row_offsets[i]) return fail(HELIX_HIP_EINVAL, name + ": decreasing row offsets");
+		for(int z = op.row_offsets[i]; z < op.row_offsets[i + 1]; ++z) {
+			const int j = op.columns[z];
+			if(j < 0 || j >= n) return fail(HELIX_HIP_EINVAL, name + ": column out of range");
+			auto& v = entries[{i, j}];
+			v.first += op.values[2 * z];
+			v.second += op.values[2 * z + 1];
+		}
+	}
+	for(const auto& e : entries) mx = std::max(mx, std::hypot(e.second.first, e.second.second));
+	for(const auto& e : entries) {
+		const auto it = entries.find({e.first.second, e.first.first});
+		const double tr = (it == entries.end()) ? 0.0 : it->second.first;
+		const double ti = (it == entries.end()) ? 0.0 : -it->second.second;
+		if(std::fabs(tr - e.second.first) > 1e-12 * mx || std::fabs(ti - e.second.second) > 1e-12 * mx)
+			return fail(HELIX_HIP_EINVAL, name + ": operator is not Hermitian");
+	}
+	return 0;
 }
 
 int validate(const helix_hip_problem* pr)
@@ -1065,6 +1170,7 @@ int validate(const helix_hip_problem* pr)
 	if(pr->layout < HELIX_HIP_LAYOUT_AUTO || pr->layout > HELIX_HIP_LAYOUT_HERMITIAN)
 		return fail(HELIX_HIP_EINVAL, "unknown layout");
 	if(pr->hamiltonian_dense) {
+		if(pr->couplings_csr) return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path needs diagonal couplings");
 		if(pr->n != 16 && pr->n != 32 && pr->n != 64)
 			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports N = 16, 32, 64");
 		for(int b = 0; b < pr->n_baths; ++b)
@@ -1081,6 +1187,13 @@ int validate(const helix_hip_problem* pr)
 		return 0;
 	};
 	if(int rc = check_op(pr->hamiltonian, "hamiltonian")) return rc;
+	if(pr->hamiltonian_csr) {
+		if(pr->hamiltonian_dense) return fail(HELIX_HIP_EINVAL, "give either a dense or a CSR Hamiltonian");
+		if(int rc = check_csr(pr->hamiltonian_csr[0], pr->n, "hamiltonian_csr")) return rc;
+	}
+	if(pr->couplings_csr)
+		for(int b = 0; b < pr->n_baths; ++b)
+			if(int rc = check_csr(pr->couplings_csr[b], pr->n, "couplings_csr[" + std::to_string(b) + "]")) return rc;
 	for(int b = 0; b < pr->n_baths; ++b) {
 		if(!pr->couplings[b].diag) return fail(HELIX_HIP_EINVAL, "coupling diag is null");
 		if(int rc = check_op(pr->couplings[b], "coupling")) return rc;
@@ -1174,6 +1287,29 @@ void launch_dense(helix_hip_plan* p, const StageArgs& a, bool last)
 		launch_dense_v<T, SPLIT, LOGN, false>(p, a, last);
 }
 
+// untiled stage: pass A ([V_b, cur] for non-diagonal couplings) and the fused RHS + epilogue
+template <typename T, bool SPLIT, int VOP, bool HCSR>
+void launch_generic(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	if constexpr(VOP != kVDiag)
+		hipLaunchKernelGGL((k_flip_commutator<T, SPLIT, VOP>), dim3(grid_for(p->elems * p->B)), dim3(kBlock), 0, p->stream,
+			a);
+	const unsigned grid = grid_for(p->elems);
+	if(last)
+		hipLaunchKernelGGL((k_stage<T, SPLIT, true, VOP, HCSR>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	else
+		hipLaunchKernelGGL((k_stage<T, SPLIT, false, VOP, HCSR>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+}
+
+template <typename T, bool SPLIT, int VOP>
+void launch_generic_h(helix_hip_plan* p, const StageArgs& a, bool last)
+{
+	if(p->csr_h)
+		launch_generic<T, SPLIT, VOP, true>(p, a, last);
+	else
+		launch_generic<T, SPLIT, VOP, false>(p, a, last);
+}
+
 template <typename T, bool SPLIT>
 void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 {
@@ -1197,17 +1333,12 @@ void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 		default: launch_tile<T, SPLIT, 10>(p, a, last); return;
 		}
 	}
-	if(p->flip) {
-		hipLaunchKernelGGL((k_flip_commutator<T, SPLIT>), dim3(grid_for(p->elems * p->B)), dim3(kBlock), 0, p->stream, a);
-		if(last)
-			hipLaunchKernelGGL((k_stage<T, SPLIT, true, true>), dim3(grid), dim3(kBlock), 0, p->stream, a);
-		else
-			hipLaunchKernelGGL((k_stage<T, SPLIT, false, true>), dim3(grid), dim3(kBlock), 0, p->stream, a);
-	} else {
-		if(last)
-			hipLaunchKernelGGL((k_stage<T, SPLIT, true, false>), dim3(grid), dim3(kBlock), 0, p->stream, a);
-		else
-			hipLaunchKernelGGL((k_stage<T, SPLIT, false, false>), dim3(grid), dim3(kBlock), 0, p->stream, a);
+	(void)grid;
+	const int vop = p->csr_v ? kVCsr : (p->flip ? kVFlip : kVDiag);
+	switch(vop) {
+	case kVCsr: launch_generic_h<T, SPLIT, kVCsr>(p, a, last); break;
+	case kVFlip: launch_generic_h<T, SPLIT, kVFlip>(p, a, last); break;
+	default: launch_generic_h<T, SPLIT, kVDiag>(p, a, last); break;
 	}
 }
 
@@ -1490,11 +1621,15 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	p->precision = pr->precision;
 	p->dt = pr->dt;
 	p->esize = (pr->precision == HELIX_HIP_FP64) ? 16 : 8;
-	for(int b = 0; b < p->B; ++b)
-		if(pr->couplings[b].nflips > 0) p->flip = true;
+	p->csr_h = pr->hamiltonian_csr != nullptr;
+	p->csr_v = pr->couplings_csr != nullptr;
+	if(!p->csr_v)
+		for(int b = 0; b < p->B; ++b)
+			if(pr->couplings[b].nflips > 0) p->flip = true;
 	{
 		const char* force = std::getenv("HELIX_HIP_GENERIC_STAGE");
-		p->tiled = !p->flip && p->B <= 16 && p->M <= kTileMaxModes && !(force && force[0] == '1');
+		p->tiled = !p->flip && !p->csr_h && !p->csr_v && p->B <= 16 && p->M <= kTileMaxModes &&
+			!(force && force[0] == '1');
 	}
 	if(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
 		return bail(fail(HELIX_HIP_ENOMEM, "hipStreamCreate failed"));
@@ -1561,7 +1696,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		a.hcoef[f] = pr->hamiltonian.flip_coefs[f];
 	}
 	// sigma^z structure: every coupling diagonal equals alpha_b (2 n_s - 1) for one spin s, no flips
-	a.sz = (p->B <= 16) ? 1 : 0;
+	a.sz = (p->B <= 16 && !p->csr_v) ? 1 : 0;
 	for(int b = 0; b < p->B && a.sz; ++b) {
 		const helix_hip_spin_operator& op = pr->couplings[b];
 		int found = -1;
@@ -1614,7 +1749,11 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		if((rc = plan_alloc(p, &r0, p->pe * 16))) return bail(rc);
 		p->rho0_f64 = static_cast<double*>(r0);
 	}
-	if(p->flip && (rc = plan_alloc(p, &p->buf, p->elems * p->esize * p->B))) return bail(rc);
+	if((p->flip || p->csr_v) && (rc = plan_alloc(p, &p->buf, p->elems * p->esize * p->B))) return bail(rc);
+	if(p->csr_h && (rc = upload_csr(p, pr->hamiltonian_csr, 1, p->N, &a.hcsr_off, &a.hcsr_col, &a.hcsr_val)))
+		return bail(rc);
+	if(p->csr_v && (rc = upload_csr(p, pr->couplings_csr, p->B, p->N, &a.vcsr_off, &a.vcsr_col, &a.vcsr_val)))
+		return bail(rc);
 	a.buf = p->buf;
 	a.n_ado = p->H;
 	a.M = p->M;
@@ -1870,7 +2009,7 @@ int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* laun
 {
 	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
 	if(device_bytes) *device_bytes = p->device_bytes;
-	if(launches_per_step) *launches_per_step = p->order * (p->flip ? 2 : 1);
+	if(launches_per_step) *launches_per_step = p->order * ((p->flip || p->csr_v) ? 2 : 1);
 	if(uses_flip_coupling) *uses_flip_coupling = p->flip ? 1 : 0;
 	if(uses_dense_hamiltonian) *uses_dense_hamiltonian = p->dense ? 1 : 0;
 	return 0;
@@ -2077,7 +2216,8 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 			return fail(HELIX_HIP_EINVAL, "batched problems must share precision and order");
 		if(pr->order < 2) return fail(HELIX_HIP_EUNSUPPORTED, "batched form needs Taylor order >= 2");
 		if(pr->n > 64) return fail(HELIX_HIP_EUNSUPPORTED, "batched form supports N <= 64");
-		if(pr->hamiltonian_dense) return fail(HELIX_HIP_EUNSUPPORTED, "batched form uses spin-structured H");
+		if(pr->hamiltonian_dense || pr->hamiltonian_csr || pr->couplings_csr)
+			return fail(HELIX_HIP_EUNSUPPORTED, "batched form uses spin-structured H and sigma^z couplings");
 		if(pr->n_modes > kTileMaxModes) return fail(HELIX_HIP_EUNSUPPORTED, "batched form supports up to 16 modes");
 	}
 	bool herm = true;

@@ -210,6 +210,24 @@ bool to_spin_operator(const SparseOperator& op, setup::SpinOperator& out, std::s
 	return true;
 }
 
+// SparseOperator (the reference's CSR) -> the set-up's CSR; the plan checks Hermiticity
+setup::CsrOperator to_setup_csr(const SparseOperator& op, std::size_t n, const std::string& name)
+{
+	if(op.rows != n || op.cols != n || op.rowOffsets.size() != n + 1 || op.columnIndices.size() != op.values.size() ||
+		op.rowOffsets.back() != op.values.size())
+		throw std::invalid_argument(name + ": malformed CSR operator");
+	if(n < 8 || (n & (n - 1)) != 0) throw std::invalid_argument(name + ": dimension must be a power of two >= 8");
+	setup::CsrOperator c;
+	c.n = static_cast<int>(n);
+	for(std::size_t v : op.rowOffsets) c.offsets.push_back(static_cast<int32_t>(v));
+	for(std::size_t v : op.columnIndices) c.columns.push_back(static_cast<int32_t>(v));
+	for(const auto& v : op.values) {
+		c.values.push_back(v.real());
+		c.values.push_back(v.imag());
+	}
+	return c;
+}
+
 setup::BathSpec bath_spec(const Bath& b, setup::SpinOperator coupling)
 {
 	setup::BathSpec s;
@@ -236,20 +254,36 @@ std::unique_ptr<setup::Problem> make_problem(const ContextOptions& o, const Syst
 	}
 	if(system->kind != SystemKind::Sparse) throw std::invalid_argument("only Sparse and LegacySpinGlass systems execute");
 	std::string why;
+	// spin-structured operators (diagonal + bit flips) take the structured kernels; any other Hermitian
+	// CSR operator runs on the generic CSR kernels
 	setup::SpinOperator ham;
-	if(!to_spin_operator(system->hamiltonian, ham, why)) throw std::invalid_argument("hamiltonian: " + why);
+	const bool h_csr = !to_spin_operator(system->hamiltonian, ham, why);
 	if(system->couplings.empty()) throw std::invalid_argument("a Sparse system needs at least one coupling operator");
 	if(h.baths.size() != 1 && h.baths.size() != system->couplings.size())
 		throw std::invalid_argument("HierarchySpec::baths must hold one bath, or one bath per coupling");
-	std::vector<setup::BathSpec> baths;
-	for(std::size_t c = 0; c < system->couplings.size(); ++c) {
-		setup::SpinOperator v;
-		if(!to_spin_operator(system->couplings[c], v, why))
-			throw std::invalid_argument("coupling[" + std::to_string(c) + "]: " + why);
-		baths.push_back(bath_spec(h.baths[h.baths.size() == 1 ? 0 : c], std::move(v)));
+	std::vector<setup::SpinOperator> vs(system->couplings.size());
+	bool v_csr = false;
+	for(std::size_t c = 0; c < system->couplings.size(); ++c)
+		if(!to_spin_operator(system->couplings[c], vs[c], why)) v_csr = true;
+	const std::size_t n = system->hamiltonian.rows;
+	std::vector<setup::CsrOperator> vcsr;
+	if(v_csr)
+		for(std::size_t c = 0; c < system->couplings.size(); ++c) {
+			vcsr.push_back(to_setup_csr(system->couplings[c], n, "coupling[" + std::to_string(c) + "]"));
+			vs[c] = vcsr.back().diagonal_part();
+		}
+	setup::CsrOperator hcsr;
+	if(h_csr) {
+		hcsr = to_setup_csr(system->hamiltonian, n, "hamiltonian");
+		ham = hcsr.diagonal_part();
 	}
-	return std::make_unique<setup::Problem>(std::move(ham), std::move(baths), static_cast<int>(h.maxDepth), order,
+	std::vector<setup::BathSpec> baths;
+	for(std::size_t c = 0; c < system->couplings.size(); ++c)
+		baths.push_back(bath_spec(h.baths[h.baths.size() == 1 ? 0 : c], std::move(vs[c])));
+	auto problem = std::make_unique<setup::Problem>(std::move(ham), std::move(baths), static_cast<int>(h.maxDepth), order,
 		o.timeStep, prec);
+	if(h_csr || v_csr) problem->set_csr_operators(h_csr ? &hcsr : nullptr, v_csr ? &vcsr : nullptr);
+	return problem;
 }
 
 }  // namespace

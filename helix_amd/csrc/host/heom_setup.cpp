@@ -411,6 +411,13 @@ Problem::Problem(SpinOperator hamiltonian, std::vector<BathSpec> baths, int tier
 void Problem::set_dense_hamiltonian(std::vector<cdouble> h)
 {
 	const size_t n2 = static_cast<size_t>(n_) * n_;
+	if(h.empty() && hcsr_.n > 0) {
+		h.assign(n2, cdouble());
+		for(int i = 0; i < n_; ++i)
+			for(int z = hcsr_.offsets[i]; z < hcsr_.offsets[i + 1]; ++z)
+				h[static_cast<size_t>(i) * n_ + hcsr_.columns[z]] +=
+					cdouble(hcsr_.values[2 * static_cast<size_t>(z)], hcsr_.values[2 * static_cast<size_t>(z) + 1]);
+	}
 	if(h.empty()) {
 		h.assign(n2, cdouble());
 		for(int i = 0; i < n_; ++i) {
@@ -425,12 +432,67 @@ void Problem::set_dense_hamiltonian(std::vector<cdouble> h)
 		hdense_[2 * q + 1] = h[q].imag();
 	}
 	desc_.hamiltonian_dense = hdense_.data();
+	desc_.hamiltonian_csr = nullptr;
 }
 
 void Problem::clear_dense_hamiltonian()
 {
 	hdense_.clear();
 	desc_.hamiltonian_dense = nullptr;
+	desc_.hamiltonian_csr = (hcsr_.n > 0) ? &hcsr_view_ : nullptr;
+}
+
+helix_hip_csr_operator CsrOperator::view() const
+{
+	helix_hip_csr_operator v{};
+	v.nnz = static_cast<int32_t>(columns.size());
+	v.row_offsets = offsets.data();
+	v.columns = columns.data();
+	v.values = values.data();
+	return v;
+}
+
+SpinOperator CsrOperator::diagonal_part() const
+{
+	SpinOperator op;
+	op.diag.assign(n, 0.0);
+	for(int i = 0; i < n; ++i)
+		for(int z = offsets[i]; z < offsets[i + 1]; ++z)
+			if(columns[z] == i) op.diag[i] += values[2 * static_cast<size_t>(z)];
+	return op;
+}
+
+void Problem::set_csr_operators(const CsrOperator* hamiltonian, const std::vector<CsrOperator>* couplings)
+{
+	if(hamiltonian) {
+		if(hamiltonian->n != n_ || static_cast<int>(hamiltonian->offsets.size()) != n_ + 1)
+			throw std::invalid_argument("CSR Hamiltonian dimension mismatch");
+		hcsr_ = *hamiltonian;
+		ham_ = hcsr_.diagonal_part();
+		hcsr_view_ = hcsr_.view();
+		desc_.hamiltonian.diag = ham_.diag.data();
+		desc_.hamiltonian.nflips = 0;
+		desc_.hamiltonian.flip_masks = nullptr;
+		desc_.hamiltonian.flip_coefs = nullptr;
+		desc_.hamiltonian_csr = &hcsr_view_;
+	}
+	if(couplings) {
+		if(couplings->size() != baths_.size()) throw std::invalid_argument("one CSR coupling per bath is required");
+		vcsr_ = *couplings;
+		vcsr_view_.clear();
+		for(size_t b = 0; b < vcsr_.size(); ++b) {
+			if(vcsr_[b].n != n_ || static_cast<int>(vcsr_[b].offsets.size()) != n_ + 1)
+				throw std::invalid_argument("CSR coupling dimension mismatch");
+			baths_[b].coupling = vcsr_[b].diagonal_part();
+			couplings_[b].diag = baths_[b].coupling.diag.data();
+			couplings_[b].nflips = 0;
+			couplings_[b].flip_masks = nullptr;
+			couplings_[b].flip_coefs = nullptr;
+			vcsr_view_.push_back(vcsr_[b].view());
+		}
+		desc_.couplings = couplings_.data();
+		desc_.couplings_csr = vcsr_view_.data();
+	}
 }
 
 double Problem::energy_diagonal(const double* rho0) const
@@ -449,6 +511,16 @@ double Problem::energy_trace(const double* rho0) const
 			for(int j = 0; j < n_; ++j) {
 				const size_t hij = 2 * (static_cast<size_t>(i) * n_ + j), rji = 2 * (static_cast<size_t>(j) * n_ + i);
 				e += hdense_[hij] * rho0[rji] - hdense_[hij + 1] * rho0[rji + 1];
+			}
+		return e;
+	}
+	if(hcsr_.n > 0) {
+		double e = 0.0;
+		for(int i = 0; i < n_; ++i)
+			for(int z = hcsr_.offsets[i]; z < hcsr_.offsets[i + 1]; ++z) {
+				const int j = hcsr_.columns[z];
+				const size_t rji = 2 * (static_cast<size_t>(j) * n_ + i);
+				e += hcsr_.values[2 * static_cast<size_t>(z)] * rho0[rji] - hcsr_.values[2 * static_cast<size_t>(z) + 1] * rho0[rji + 1];
 			}
 		return e;
 	}

@@ -80,6 +80,16 @@ struct SplitMix64 {
 	double normal();
 };
 
+// A general sparse operator in CSR form (complex values interleaved).
+struct CsrOperator {
+	int n = 0;
+	std::vector<int32_t> offsets;  // [n+1]
+	std::vector<int32_t> columns;
+	std::vector<double> values;    // [nnz][2]
+	helix_hip_csr_operator view() const;
+	SpinOperator diagonal_part() const;  // Re O_ii, no flips
+};
+
 struct BathSpec {
 	double beta = 3.0, gamma = 1.0, zeta = 0.01, omega0 = 1.0;
 	int K = 2;
@@ -108,6 +118,9 @@ public:
 	void set_dense_hamiltonian(std::vector<cdouble> h = {});
 	void clear_dense_hamiltonian();
 	bool dense_hamiltonian() const { return !hdense_.empty(); }
+	// General sparse operators: a CSR H replaces the spin-structured one, CSR couplings (one per bath)
+	// replace the baths' spin-structured couplings.  Either may be null.
+	void set_csr_operators(const CsrOperator* hamiltonian, const std::vector<CsrOperator>* couplings);
 
 private:
 	int n_ = 0;
@@ -118,6 +131,10 @@ private:
 	std::vector<double> down_comm_, down_anti_, self_, xi_, xic_;
 	std::vector<helix_hip_spin_operator> couplings_;
 	std::vector<double> hdense_;  // [N][N][2] when the dense path is selected
+	CsrOperator hcsr_;            // n == 0 when H is spin-structured
+	std::vector<CsrOperator> vcsr_;
+	helix_hip_csr_operator hcsr_view_{};
+	std::vector<helix_hip_csr_operator> vcsr_view_;
 	helix_hip_problem desc_{};
 };
 

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <stdexcept>
 #include <string>
 
 using namespace helix::setup;
@@ -92,6 +93,53 @@ int helix_model_spin_glass_random(uint64_t seed, int order, double dt, int preci
 	if(max_tier_out) *max_tier_out = max_tier;
 	const std::vector<double> l(spins, lam), g(spins, gamma), t(spins, temp);
 	return helix_model_spin_glass(spins, seed, 1.0, l.data(), g.data(), t.data(), 1, max_tier, order, dt, precision, out);
+}
+
+int helix_model_sparse(int n, int h_nnz, const int32_t* h_offsets, const int32_t* h_columns, const double* h_values,
+	int n_couplings, const int32_t* v_nnz, const int32_t* v_offsets, const int32_t* v_columns, const double* v_values,
+	const double* beta, const double* gamma, const double* zeta, int pade_terms, int max_tier, int order, double dt,
+	int precision, helix_model** out)
+{
+	if(!out || !h_offsets || !v_nnz || !v_offsets || !beta || !gamma || !zeta || n_couplings <= 0)
+		return setup_fail(HELIX_HIP_EINVAL, "null argument");
+	*out = nullptr;
+	if(n < 8 || (n & (n - 1)) != 0) return setup_fail(HELIX_HIP_EINVAL, "n must be a power of two >= 8");
+	try {
+		auto csr = [n](int nnz, const int32_t* off, const int32_t* col, const double* val) {
+			CsrOperator o;
+			o.n = n;
+			o.offsets.assign(off, off + n + 1);
+			if(o.offsets.front() != 0 || o.offsets.back() != nnz) throw std::invalid_argument("bad CSR row offsets");
+			if(nnz > 0) {
+				o.columns.assign(col, col + nnz);
+				o.values.assign(val, val + 2 * static_cast<size_t>(nnz));
+			}
+			return o;
+		};
+		const CsrOperator h = csr(h_nnz, h_offsets, h_columns, h_values);
+		std::vector<CsrOperator> v;
+		std::vector<BathSpec> baths;
+		size_t base = 0;
+		for(int c = 0; c < n_couplings; ++c) {
+			v.push_back(csr(v_nnz[c], v_offsets + static_cast<size_t>(c) * (n + 1), v_columns + base, v_values + 2 * base));
+			base += static_cast<size_t>(v_nnz[c]);
+			BathSpec b;
+			b.beta = beta[c];
+			b.gamma = gamma[c];
+			b.zeta = zeta[c];
+			b.omega0 = 1.0;
+			b.K = pade_terms;
+			b.coupling = v.back().diagonal_part();
+			baths.push_back(std::move(b));
+		}
+		auto model = std::make_unique<helix_model>();
+		model->problem = std::make_unique<Problem>(h.diagonal_part(), std::move(baths), max_tier + 1, order, dt, precision);
+		model->problem->set_csr_operators(&h, &v);
+		*out = model.release();
+		return 0;
+	} catch(const std::exception& e) {
+		return setup_fail(HELIX_HIP_EINVAL, e.what());
+	}
 }
 
 int helix_model_use_dense_hamiltonian(helix_model* model, int enable)

@@ -9,8 +9,9 @@
 //  - Precision::SplitSingle (added) -> FP32 form with two-term FP32 operator values.
 //  - SystemKind::LegacySpinGlass via helix::examples::legacy_spin_glass_system(): the reference's
 //    compiled model, any Drude-Lorentz/Pade bath values and hierarchy depth.
-//  - SystemKind::Sparse executes when H and every coupling are spin-structured
-//    (diagonal + bit-flip terms with one coefficient per flip mask); one bath per coupling.
+//  - SystemKind::Sparse executes for any Hermitian H and couplings (one bath per coupling):
+//    spin-structured operators (diagonal + bit-flip terms with one coefficient per flip mask) run on
+//    the structured kernels, every other CSR operator on the generic sparse kernels.
 //  - ResultMode::FinalState and ResultMode::ObservableTrace execute; Trajectory is validation-only.
 //  - Backend::CpuReference is unsupported (no CPU execution path ships in the library).
 

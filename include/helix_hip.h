@@ -72,6 +72,15 @@ typedef struct helix_hip_spin_operator {
 	const double* flip_coefs;    /* [nflips] real coefficient of each flip */
 } helix_hip_spin_operator;
 
+/* A general sparse Hermitian operator in CSR form (the reference's SparseOperator,
+ * include/helix/types.h:100-106, with int32 indices); values are complex [nnz][2]. */
+typedef struct helix_hip_csr_operator {
+	int32_t nnz;
+	const int32_t* row_offsets;  /* [n+1] */
+	const int32_t* columns;      /* [nnz], 0 <= column < n */
+	const double* values;        /* [nnz][2] */
+} helix_hip_csr_operator;
+
 typedef struct helix_hip_problem {
 	int32_t abi_version;         /* HELIX_HIP_ABI_VERSION */
 	int32_t n;                   /* Hilbert dimension N, a power of two, 8 <= N <= 4096 */
@@ -94,6 +103,11 @@ typedef struct helix_hip_problem {
 	int32_t order;               /* Taylor order m, 1 <= m <= 8 (ContextOptions::integrationOrder) */
 	double dt;                   /* time step (ContextOptions::timeStep) */
 	int32_t layout;              /* enum helix_hip_layout */
+	/* optional general sparse operators (each must be Hermitian): a non-null hamiltonian_csr replaces
+	 * `hamiltonian`, a non-null couplings_csr [B] replaces `couplings` (whose diagonals must still be given,
+	 * e.g. the CSR diagonal).  Such problems run on the generic two-pass stage kernels. */
+	const helix_hip_csr_operator* hamiltonian_csr;
+	const helix_hip_csr_operator* couplings_csr;
 } helix_hip_problem;
 
 typedef struct helix_hip_plan helix_hip_plan;

@@ -36,6 +36,17 @@ int helix_model_spin_glass(int spins, uint64_t seed, double h, const double* lam
 int helix_model_spin_glass_random(uint64_t seed, int order, double dt, int precision, helix_model** out,
 	int* spins_out, int* max_tier_out);
 
+/* A general sparse system (System::from_sparse, include/helix/types.h:145-153): a Hermitian H and
+ * n_couplings Hermitian couplings in CSR form (int32 indices, complex values [nnz][2]), one Drude-Lorentz /
+ * Pade bath per coupling given by the reference's Bath fields (inverse temperature beta, damping gamma,
+ * coupling strength zeta; lambda = zeta gamma / 2), tiers 0..max_tier.  The couplings are concatenated:
+ * v_offsets [n_couplings][n+1] (each relative to its own operator), v_nnz [n_couplings], v_columns and
+ * v_values the operators' arrays one after the other.  Runs on the generic CSR stage kernels. */
+int helix_model_sparse(int n, int h_nnz, const int32_t* h_offsets, const int32_t* h_columns, const double* h_values,
+	int n_couplings, const int32_t* v_nnz, const int32_t* v_offsets, const int32_t* v_columns, const double* v_values,
+	const double* beta, const double* gamma, const double* zeta, int pade_terms, int max_tier, int order, double dt,
+	int precision, helix_model** out);
+
 /* Selects (enable = 1) or deselects the dense MFMA commutator for H, materialising the model's
  * Hamiltonian as [N][N] complex (N must be 16, 32 or 64 when the plan is created). */
 int helix_model_use_dense_hamiltonian(helix_model* model, int enable);
