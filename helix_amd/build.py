@@ -62,6 +62,16 @@ def build(verbose: bool = False) -> dict:
           "-Wl,-rpath,$ORIGIN"])
     out["libhelix"] = host_lib
 
+    # the `helix` Python module: pybind11 bindings of the public C++ API
+    import sysconfig
+    import pybind11
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    pymod = os.path.join(PKG, "helix" + ext)
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{INCLUDE}", f"-I{pybind11.get_include()}",
+          f"-I{sysconfig.get_paths()['include']}", os.path.join(CSRC, "host", "python_module.cpp"), "-o", pymod,
+          f"-L{PKG}", "-lhelix", "-lhelix_hip", "-Wl,-rpath,$ORIGIN"])
+    out["helix_python_module"] = pymod
+
     for src, name in (("legacy_spin_glass.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint"),
                       ("sparse_system.cpp", "helix_sparse_system")):
         example = os.path.join(ROOT, "examples", src)
