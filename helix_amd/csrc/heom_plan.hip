@@ -374,6 +374,42 @@ __device__ __forceinline__ void rmadd(T& ar, T& ai, double w, T xr, T xi)
 	}
 }
 
+// A real operator weight in the form the FMAs consume: FP64 as is, FP32 rounded, split-FP32 as the two-term
+// sum hi + lo.
+template <typename T, bool SPLIT>
+struct Weight {
+	T hi;
+	T lo;
+	__device__ __forceinline__ static Weight make(double w)
+	{
+		Weight r;
+		r.hi = static_cast<T>(w);
+		r.lo = SPLIT ? static_cast<T>(w - static_cast<double>(r.hi)) : T(0);
+		return r;
+	}
+	__device__ __forceinline__ Weight select(bool take_other, const Weight& other, bool negate) const
+	{
+		Weight r{take_other ? other.hi : hi, take_other ? other.lo : lo};
+		if(negate) {
+			r.hi = -r.hi;
+			r.lo = -r.lo;
+		}
+		return r;
+	}
+};
+
+// acc += w * x with a prepared weight (same operation order as rmadd)
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void wmadd(T& ar, T& ai, const Weight<T, SPLIT>& w, T xr, T xi)
+{
+	if constexpr(SPLIT) {
+		ar += w.lo * xr;
+		ai += w.lo * xi;
+	}
+	ar += w.hi * xr;
+	ai += w.hi * xi;
+}
+
 // LDS a tile body needs (sized for the largest tile that uses it)
 template <typename T>
 struct TileLds {
@@ -382,7 +418,9 @@ struct TileLds {
 	double* v;      // [16 N]   general diagonal couplings only
 	double2* selfx; // [N]      sigma^z couplings only
 	int* nb;        // [A][2 kTileMaxModes]
-	int* mb;        // [kTileMaxModes] bath of each mode
+	int* mb;        // [kTileMaxModes] per mode: the bath (general couplings) or its sigma^z bit mask (SZ)
+	double* ka;     // [kTileMaxModes] per mode, SZ: 2 alpha of its bath
+	double* hd;     // [N] diagonal of H
 };
 
 // Register slot r of a tile thread (lt = lane within its ADO): the stored element q within the ADO and
@@ -431,6 +469,8 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	double2* __restrict__ s_selfx = lds.selfx;
 	int (*s_nb)[2 * kTileMaxModes] = reinterpret_cast<int (*)[2 * kTileMaxModes]>(lds.nb);
 	int* __restrict__ s_mb = lds.mb;
+	double* __restrict__ s_ka = lds.ka;
+	double* __restrict__ s_hd = lds.hd;
 
 	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
@@ -449,7 +489,16 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	} else {
 		for(int q = tid; q < B * N; q += THREADS) s_v[q] = a.vdiag[q];
 	}
-	for(int q = tid; q < M; q += THREADS) s_mb[q] = a.mode_bath[q];
+	for(int q = tid; q < M; q += THREADS) {
+		const int b = a.mode_bath[q];
+		if constexpr(SZ) {
+			s_mb[q] = a.sz_mask[b];
+			s_ka[q] = 2.0 * a.sz_alpha[b];
+		} else {
+			s_mb[q] = b;
+		}
+	}
+	for(int q = tid; q < N; q += THREADS) s_hd[q] = a.hdiag[q];
 	for(int q = tid; q < A * 2 * M; q += THREADS) {
 		const int s = q / (2 * M), k = q - s * (2 * M);
 		const int nn = (group / TPA) * A + s;
@@ -497,7 +546,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		const int le = e - ebase;
 		const V2 c = blk[le];
 		T hr = T(0), hi = T(0);
-		rmadd<T, SPLIT>(hr, hi, a.hdiag[i] - a.hdiag[j], c.x, c.y);
+		rmadd<T, SPLIT>(hr, hi, s_hd[i] - s_hd[j], c.x, c.y);
 		for(int f = 0; f < a.hflips; ++f) {
 			const int m = a.hmask[f];
 			const int rf = m << LOGN;  // row flip stays in the tile unless it crosses tiles
@@ -514,7 +563,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// before any arithmetic, so a thread waits for one memory round trip per NB neighbours -- small
 	// problems, with few workgroups in flight, are bound by that chain.
 	// (sized to what each variant's register budget holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
-	constexpr int NB = HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : (THREADS <= 512 ? 4 : 2))
+	constexpr int NB = HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
 	                        : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
 	for(int q0 = 0; q0 < 2 * M; q0 += NB) {
 		int nid[NB];
@@ -546,14 +595,16 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		for(int u = 0; u < NB; ++u) {
 			if(nid[u] < 0) continue;
 			const int q = q0 + u;
-			const int b = s_mb[q >= M ? q - M : q];
+			const int k = q >= M ? q - M : q;
 			if constexpr(SZ) {
 				// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
 				// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
 				// Up: (-i) d x.  Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
-				const int mask = a.sz_mask[b];
-				const double two_alpha = 2.0 * a.sz_alpha[b];
-				const double wcomm = wc[u].y, wanti = wa[u].x;
+				// The two weights 2 alpha wc.y and 2 alpha wa.x are formed once; bit i only flips the sign.
+				const int mask = s_mb[k];
+				const double two_alpha = s_ka[k];
+				const Weight<T, SPLIT> wd = Weight<T, SPLIT>::make(two_alpha * wc[u].y);
+				const Weight<T, SPLIT> ws = Weight<T, SPLIT>::make(two_alpha * wa[u].x);
 #pragma unroll
 				for(int r = 0; r < RT; ++r) {
 					if(!HX_SLOT_LIVE(r)) continue;
@@ -561,14 +612,14 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
 					const int bi = i & mask, bj = j & mask;
 					const bool differ = bi != bj;
-					const double sg = bi ? two_alpha : -two_alpha;
-					const double w = sg * (differ ? wcomm : wanti);
+					const Weight<T, SPLIT> w = ws.select(differ, wd, bi == 0);
 					// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
 					const T re = differ ? -x[u][r].y : x[u][r].x;
 					const T im = differ ? x[u][r].x : x[u][r].y;
-					rmadd<T, SPLIT>(ar[r], ai[r], w, re, im);
+					wmadd<T, SPLIT>(ar[r], ai[r], w, re, im);
 				}
 			} else {
+				const int b = s_mb[k];
 				const double* __restrict__ vb = s_v + b * N;
 #pragma unroll
 				for(int r = 0; r < RT; ++r) {
@@ -684,11 +735,14 @@ k_stage_tile(const StageArgs a, int ngroups)
 	__shared__ double2 s_selfx[SZ ? N : 1];
 	__shared__ int s_nb[A * 2 * kTileMaxModes];
 	__shared__ int s_mb[kTileMaxModes];
+	__shared__ double s_ka[kTileMaxModes];
+	__shared__ double s_hd[N];
 	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
 	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
 	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(group >= ngroups) return;
-	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group, TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_mb});
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group,
+		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_mb, s_ka, s_hd});
 }
 
 // Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
@@ -707,12 +761,14 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	__shared__ double2 s_selfx[1 << kBatchMaxLogN];
 	__shared__ int s_nb[(kTile >> 6) * 2 * kTileMaxModes];
 	__shared__ int s_mb[kTileMaxModes];
+	__shared__ double s_ka[kTileMaxModes];
+	__shared__ double s_hd[1 << kBatchMaxLogN];
 	const int per_xcd = (ntasks + kXcds - 1) / kXcds;
 	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
 	const StageArgs& a = table[tk.x];
-	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_mb};
+	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_mb, s_ka, s_hd};
 	switch(a.logN) {
 	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
 	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
