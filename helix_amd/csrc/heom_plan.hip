@@ -417,7 +417,8 @@ struct TileLds {
 	V2* blk;        // [kTile]
 	double* v;      // [16 N]   general diagonal couplings only
 	double2* selfx; // [N]      sigma^z couplings only
-	int* nb;        // [A][2 kTileMaxModes]
+	int* nb;        // [A][2 kTileMaxModes] valid neighbours of each ADO, packed (index << 5 | slot q)
+	int* cnt;       // [A] number of valid neighbours
 	int* mb;        // [kTileMaxModes] per mode: the bath (general couplings) or its sigma^z bit mask (SZ)
 	double* ka;     // [kTileMaxModes] per mode, SZ: 2 alpha of its bath
 	double* hd;     // [N] diagonal of H
@@ -468,6 +469,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	double* __restrict__ s_v = lds.v;
 	double2* __restrict__ s_selfx = lds.selfx;
 	int (*s_nb)[2 * kTileMaxModes] = reinterpret_cast<int (*)[2 * kTileMaxModes]>(lds.nb);
+	int* __restrict__ s_cnt = lds.cnt;
 	int* __restrict__ s_mb = lds.mb;
 	double* __restrict__ s_ka = lds.ka;
 	double* __restrict__ s_hd = lds.hd;
@@ -499,15 +501,24 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		}
 	}
 	for(int q = tid; q < N; q += THREADS) s_hd[q] = a.hdiag[q];
-	for(int q = tid; q < A * 2 * M; q += THREADS) {
-		const int s = q / (2 * M), k = q - s * (2 * M);
+	// compacted neighbour lists: lanes 32s' .. 32s'+31 of a wave hold the 2M <= 32 candidate slots q of one ADO;
+	// a ballot gives each valid neighbour its position, so the stage loop visits valid entries only (the
+	// deepest tier, most of the hierarchy, has no up-neighbours)
+	static_assert(2 * kTileMaxModes == 32 && THREADS % 64 == 0, "one half-wave per ADO slot list");
+	for(int base = 0; base < A * 32; base += THREADS) {
+		const int t = base + tid;
+		const int s = t >> 5, k = t & 31;
 		const int nn = (group / TPA) * A + s;
 		int idx = -1;
-		if(nn < H) {
+		if(s < A && k < 2 * M && nn < H) {
 			const int v = (k < M) ? a.up[nn * M + k] : a.down[nn * M + (k - M)];
 			idx = (v < H) ? v : -1;
 		}
-		s_nb[s][k] = idx;
+		const unsigned long long bal = __ballot(idx >= 0);
+		const int lane = tid & 63;
+		const unsigned long long half = (lane < 32) ? 0xffffffffull : 0xffffffff00000000ull;
+		if(idx >= 0) s_nb[s][__popcll(bal & half & ((1ull << lane) - 1ull))] = (idx << 5) | k;
+		if(k == 0 && s < A) s_cnt[s] = __popcll(bal & half);
 	}
 	if constexpr(HERM) {
 		if(live) {
@@ -565,14 +576,17 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// (sized to what each variant's register budget holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
 	constexpr int NB = HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
 	                        : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
-	for(int q0 = 0; q0 < 2 * M; q0 += NB) {
-		int nid[NB];
+	const int nvalid = s_cnt[slot];
+	for(int c0 = 0; c0 < nvalid; c0 += NB) {
+		int nid[NB], nq[NB];
 		double2 wc[NB], wa[NB];
 		V2 x[NB][RT];
 #pragma unroll
 		for(int u = 0; u < NB; ++u) {
-			const int q = q0 + u;
-			nid[u] = (q < 2 * M) ? s_nb[slot][q] : -1;
+			const int packed = (c0 + u < nvalid) ? s_nb[slot][c0 + u] : -1;
+			nid[u] = packed >> 5;  // -1 when past the list
+			nq[u] = packed & 31;
+			const int q = nq[u];
 			wc[u] = make_double2(0.0, -1.0);  // up: weight -i on the commutator
 			wa[u] = make_double2(0.0, 0.0);
 #pragma unroll
@@ -594,7 +608,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 #pragma unroll
 		for(int u = 0; u < NB; ++u) {
 			if(nid[u] < 0) continue;
-			const int q = q0 + u;
+			const int q = nq[u];
 			const int k = q >= M ? q - M : q;
 			if constexpr(SZ) {
 				// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
@@ -734,6 +748,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 	__shared__ double s_v[SZ ? 1 : kTileMaxModes * N];
 	__shared__ double2 s_selfx[SZ ? N : 1];
 	__shared__ int s_nb[A * 2 * kTileMaxModes];
+	__shared__ int s_cnt[A];
 	__shared__ int s_mb[kTileMaxModes];
 	__shared__ double s_ka[kTileMaxModes];
 	__shared__ double s_hd[N];
@@ -742,7 +757,7 @@ k_stage_tile(const StageArgs a, int ngroups)
 	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(group >= ngroups) return;
 	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group,
-		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_mb, s_ka, s_hd});
+		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd});
 }
 
 // Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
@@ -760,6 +775,7 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	__shared__ double s_v[1];
 	__shared__ double2 s_selfx[1 << kBatchMaxLogN];
 	__shared__ int s_nb[(kTile >> 6) * 2 * kTileMaxModes];
+	__shared__ int s_cnt[kTile >> 6];
 	__shared__ int s_mb[kTileMaxModes];
 	__shared__ double s_ka[kTileMaxModes];
 	__shared__ double s_hd[1 << kBatchMaxLogN];
@@ -768,7 +784,7 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
 	const StageArgs& a = table[tk.x];
-	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_mb, s_ka, s_hd};
+	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd};
 	switch(a.logN) {
 	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
 	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
