@@ -545,28 +545,38 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	T ar[RT], ai[RT];
 	const V2* __restrict__ blk = s_blk + slot * TE;
 	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);  // full layout, N > 64 only
-	// -i[H, rho_n] from LDS
+	// -i[H, rho_n] from LDS: the diagonal, then flip by flip (each coefficient prepared once for all elements)
+	T hr[RT], hi[RT];
 #pragma unroll
 	for(int r = 0; r < RT; ++r) {
-		ar[r] = T(0);
-		ai[r] = T(0);
+		hr[r] = T(0);
+		hi[r] = T(0);
 		if(!HX_SLOT_LIVE(r)) continue;
 		int q, i, j;
 		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
-		const int e = (i << LOGN) | j;
-		const int le = e - ebase;
-		const V2 c = blk[le];
-		T hr = T(0), hi = T(0);
-		rmadd<T, SPLIT>(hr, hi, s_hd[i] - s_hd[j], c.x, c.y);
-		for(int f = 0; f < a.hflips; ++f) {
-			const int m = a.hmask[f];
-			const int rf = m << LOGN;  // row flip stays in the tile unless it crosses tiles
+		const V2 c = blk[((i << LOGN) | j) - ebase];
+		rmadd<T, SPLIT>(hr[r], hi[r], s_hd[i] - s_hd[j], c.x, c.y);
+	}
+	for(int f = 0; f < a.hflips; ++f) {
+		const int m = a.hmask[f];
+		const int rf = m << LOGN;  // row flip stays in the tile unless it crosses tiles
+		const Weight<T, SPLIT> w = Weight<T, SPLIT>::make(a.hcoef[f]);
+#pragma unroll
+		for(int r = 0; r < RT; ++r) {
+			if(!HX_SLOT_LIVE(r)) continue;
+			int q, i, j;
+			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+			const int e = (i << LOGN) | j;
+			const int le = e - ebase;
 			const V2 x1 = (TPA == 1 || rf < kTile) ? blk[le ^ rf] : own[e ^ rf];
 			const V2 x2 = blk[le ^ m];
-			rmadd<T, SPLIT>(hr, hi, a.hcoef[f], x1.x - x2.x, x1.y - x2.y);
+			wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x - x2.x, x1.y - x2.y);
 		}
-		ar[r] = hi;
-		ai[r] = -hr;
+	}
+#pragma unroll
+	for(int r = 0; r < RT; ++r) {
+		ar[r] = hi[r];
+		ai[r] = -hr[r];
 	}
 
 	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}},
