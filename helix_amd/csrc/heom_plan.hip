@@ -56,19 +56,27 @@ struct C2t<float> {
 };
 
 // Hermitian layout (include/helix_hip.h): stored index of full element (i, j) of an N x N block, and
-// whether the stored value must be conjugated.  Main [N/2][N]: (d, i) = rho[i][(i + d) mod N] for
-// d < N/2; tail [N/2]: t = rho[t][t + N/2]; other elements are conjugates of stored ones.
+// whether the stored value must be conjugated.  The stored elements are rho[i][(i + d) mod N] for d < N/2
+// (the main part) and rho[t][t + N/2] for t < N/2 (the tail, after the main part); every other element
+// is the conjugate of a stored one.  The main part is ordered d-major, [N/2][N] with element (d, i), for
+// N <= 64 (a lane walk over i at fixed d is contiguous and a workgroup holds whole blocks), and i-major,
+// [N][N/2] with element (i, d), for N > 64 (a band of rows is one contiguous slice, as the row tiles read it).
+__host__ __device__ inline int herm_main(int i, int d, int logN)
+{
+	return (logN <= 6) ? ((d << logN) | i) : ((i << (logN - 1)) | d);
+}
+
 __host__ __device__ inline int herm_index(int i, int j, int logN, bool& conj)
 {
 	const int N = 1 << logN;
 	const int d = (j - i) & (N - 1);
 	if(d < N / 2) {
 		conj = false;
-		return (d << logN) | i;
+		return herm_main(i, d, logN);
 	}
 	if(d > N / 2) {
 		conj = true;
-		return ((N - d) << logN) | j;
+		return herm_main(j, N - d, logN);
 	}
 	conj = i >= N / 2;
 	return N * N / 2 + (conj ? j : i);
@@ -432,7 +440,7 @@ template <int LOGN, int TA, int EPT, bool HERM>
 __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int& i, int& j)
 {
 	constexpr int N = 1 << LOGN;
-	if constexpr(HERM) {
+	if constexpr(HERM && LOGN <= 6) {
 		if(r < EPT) {
 			q = lt + TA * r;
 			i = q & (N - 1);
@@ -441,6 +449,19 @@ __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int&
 			q = N * N / 2 + lt;
 			i = lt;
 			j = lt + N / 2;
+		}
+	} else if constexpr(HERM) {
+		// row band of R = 4096/N rows from r0 = ebase / N: its stored main part is one contiguous slice
+		const int r0 = ebase >> LOGN;
+		if(r < EPT) {
+			const int qq = lt + TA * r;  // 0 .. 2047
+			q = (r0 << (LOGN - 1)) + qq;
+			i = r0 + (qq >> (LOGN - 1));
+			j = (i + (qq & (N / 2 - 1))) & (N - 1);
+		} else {
+			i = r0 + lt;
+			q = N * N / 2 + i;
+			j = i + N / 2;
 		}
 	} else {
 		q = ebase + lt + TA * r;
@@ -456,15 +477,15 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	constexpr int N = 1 << LOGN;
 	constexpr int LOGE = 2 * LOGN;
 	constexpr int E = 1 << LOGE;
-	static_assert(!HERM || E <= kTile, "the Hermitian layout tiles whole ADO blocks (N <= 64)");
 	constexpr int TPA = (E > kTile) ? E / kTile : 1;  // tiles per ADO (N > 64: row tiles of one block)
+	constexpr int R = kTile >> LOGN;                  // rows of a row tile (N > 64)
 	constexpr int A = (E >= kTile) ? 1 : kTile / E;   // ADOs per workgroup (N < 64)
 	constexpr int TE = (E >= kTile) ? kTile : E;      // full-block elements of one ADO held in LDS
 	constexpr int TA = THREADS / A;                   // threads per ADO
 	constexpr int EPT = (HERM ? TE / 2 : TE) / TA;    // main elements per thread
 	constexpr int RT = HERM ? EPT + 1 : EPT;          // register slots per thread (+ the tail element)
 	constexpr size_t PE = HERM ? (static_cast<size_t>(E) + N) / 2 : static_cast<size_t>(E);  // stored per ADO
-	static_assert(!HERM || TA >= N / 2, "every tail element needs a thread");
+	static_assert(!HERM || TA >= (TPA == 1 ? N / 2 : R), "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
 	double2* __restrict__ s_selfx = lds.selfx;
@@ -482,8 +503,8 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	const int lt = tid - slot * TA;
 	const int n = (group / TPA) * A + slot;  // this thread's ADO (uniform over TA threads)
 	const bool live = n < H;
-	const bool tail_ok = HERM && lt < N / 2;
-	// slot r holds an element: every main slot, the tail slot only for lt < N/2
+	// the tail slot holds an element for lt < N/2 (whole blocks), or for lt < R in row tiles of the upper half
+	const bool tail_ok = HERM && (TPA == 1 ? lt < N / 2 : (lt < R && ebase < E / 2));
 #define HX_SLOT_LIVE(r) (!HERM || (r) < EPT || tail_ok)
 
 	if constexpr(SZ) {
@@ -520,7 +541,33 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		if(idx >= 0) s_nb[s][__popcll(bal & half & ((1ull << lane) - 1ull))] = (idx << 5) | k;
 		if(k == 0 && s < A) s_cnt[s] = __popcll(bal & half);
 	}
-	if constexpr(HERM) {
+	if constexpr(HERM && TPA > 1) {
+		// row band: pass 1 stores the band's own elements (one contiguous slice); pass 2 fills the rest of its
+		// rows with conjugates of elements stored in other rows, lanes walking the band rows at a fixed
+		// column so that the loads of one column are contiguous
+		if(live) {
+			const int r0 = ebase >> LOGN;
+			const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;
+#pragma unroll
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				int q, i, j;
+				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+				s_blk[((i - r0) << LOGN) | j] = ownh[q];
+			}
+			for(int t = tid; t < kTile; t += THREADS) {
+				const int c = t / R, il = t - (t / R) * R;
+				const int i = r0 + il;
+				const int d = (c - i) & (N - 1);
+				if(d < N / 2 || (d == N / 2 && i < N / 2)) continue;  // stored in row i: pass 1
+				bool cj;
+				const int src = herm_index(i, c, LOGN, cj);
+				V2 v = ownh[src];
+				if(cj) v.y = -v.y;
+				s_blk[(il << LOGN) | c] = v;
+			}
+		}
+	} else if constexpr(HERM) {
 		if(live) {
 #pragma unroll
 			for(int r = 0; r < RT; ++r) {
@@ -545,6 +592,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	T ar[RT], ai[RT];
 	const V2* __restrict__ blk = s_blk + slot * TE;
 	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);  // full layout, N > 64 only
+	const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;    // Hermitian layout, N > 64 only
 	// -i[H, rho_n] from LDS: the diagonal, then flip by flip (each coefficient prepared once for all elements)
 	T hr[RT], hi[RT];
 #pragma unroll
@@ -568,7 +616,16 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
 			const int e = (i << LOGN) | j;
 			const int le = e - ebase;
-			const V2 x1 = (TPA == 1 || rf < kTile) ? blk[le ^ rf] : own[e ^ rf];
+			V2 x1;
+			if(TPA == 1 || rf < kTile) {
+				x1 = blk[le ^ rf];
+			} else if constexpr(HERM) {  // row i^m lies outside the band: its element from the packed block
+				bool cj;
+				x1 = ownh[herm_index(i ^ m, j, LOGN, cj)];
+				if(cj) x1.y = -x1.y;
+			} else {
+				x1 = own[e ^ rf];
+			}
 			const V2 x2 = blk[le ^ m];
 			wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x - x2.x, x1.y - x2.y);
 		}
@@ -1331,7 +1388,7 @@ void launch_tile_sz(helix_hip_plan* p, const StageArgs& a, bool last)
 			return;
 		}
 	}
-	if constexpr(LOGN <= 6) {
+	if constexpr(LOGN <= 6 || SZ) {
 		if(p->herm) {
 			launch_tile_v<T, SPLIT, LOGN, THREADS, SZ, true>(p, a, last);
 			return;
@@ -1896,10 +1953,10 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	if(p->dense) p->tiled = false;
 	if(p->logN > 10) p->tiled = false;
 
-	// storage layout: Hermitian where the tile kernel runs whole blocks (N <= 64) and L preserves it
-	const bool can_herm = p->tiled && p->logN <= 6 && preserves_hermiticity(pr);
+	// storage layout: Hermitian wherever the tile kernel runs and L preserves it
+	const bool can_herm = p->tiled && preserves_hermiticity(pr);
 	if(pr->layout == HELIX_HIP_LAYOUT_HERMITIAN && !can_herm)
-		return bail(fail(HELIX_HIP_EUNSUPPORTED, "Hermitian layout needs the tiled stage (diagonal couplings, N <= 64) "
+		return bail(fail(HELIX_HIP_EUNSUPPORTED, "Hermitian layout needs the tiled stage (diagonal couplings) "
 			"and Hermiticity-preserving coefficients"));
 	p->herm = can_herm && pr->layout != HELIX_HIP_LAYOUT_FULL;
 	const size_t n2 = static_cast<size_t>(p->N) * p->N;

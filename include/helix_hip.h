@@ -53,8 +53,9 @@ enum helix_hip_precision {
  * KXiCounter imaginary, the Hamiltonian and couplings Hermitian, then L maps Hermitian ADOs to
  * Hermitian ADOs and a Hermitian initial state keeps every ADO Hermitian.  The Hermitian layout
  * stores N(N+1)/2 elements per ADO instead of N^2 (about half the HBM traffic):
- *   main [N/2][N]: element (d, i) = rho[i][(i + d) mod N], 0 <= d < N/2
- *   tail [N/2]:    element t      = rho[t][t + N/2]
+ *   main (N^2/2):  rho[i][(i + d) mod N] for 0 <= d < N/2, as [N/2][N] element (d, i) for N <= 64
+ *                  and as [N][N/2] element (i, d) for N > 64
+ *   tail [N/2]:    element t = rho[t][t + N/2]
  * every other element is the conjugate of a stored one.  States passed in must then be Hermitian
  * (to 1e-12 relative), else the call returns HELIX_HIP_EINVAL; states passed out are full blocks. */
 enum helix_hip_layout {

@@ -31,7 +31,7 @@ def _rel(a, b):
 def test_layout_selection(hx):
     assert hx.Plan(hx.Model.spin_glass(3, 0, L=2)).layout() == "hermitian"
     assert hx.Plan(hx.Model.spin_glass(6, 1, L=1)).layout() == "hermitian"
-    assert hx.Plan(hx.Model.spin_glass(7, 2, L=1)).layout() == "full"           # N = 128: row tiles
+    assert hx.Plan(hx.Model.spin_glass(7, 2, L=1)).layout() == "hermitian"      # N = 128: row tiles
     assert hx.Plan(hx.Model.legacy(N=32)).layout() == "full"                     # bit-flip coupling
     assert hx.Plan(hx.Model.spin_glass(4, 5, L=2).use_dense_hamiltonian()).layout() == "full"
     with pytest.raises(hx.HelixError):
@@ -58,6 +58,29 @@ def test_hermitian_equals_full_layout_trajectory(hx, oracle_mod, spins, L):
         assert _rel(out["hermitian"], ref) <= FP64_TOL
     h = out["hermitian"]
     assert np.max(np.abs(h - np.conj(np.transpose(h, (0, 2, 1))))) == 0.0  # unpacked state is Hermitian
+
+
+@pytest.mark.parametrize("spins,L,steps", [(7, 2, 40), (8, 1, 20), (8, 2, 5)])
+def test_hermitian_row_tiles_equal_full_layout(hx, spins, L, steps):
+    """N = 128, 256 (the tiled kernel's 16-mode limit caps the spin glass at 8 spins): row tiles of the
+    Hermitian layout (each band's rows completed in LDS from the transposed rows, far row flips gathered
+    from the packed block) against the full layout."""
+    m = hx.Model.spin_glass(spins, 2, L=L, dt=0.01, lam=0.3, gamma=0.7, T=1.3)
+    s0 = _rand_state(m.H, m.N, 17) * 1e-2
+    out = {}
+    for layout in ("hermitian", "full"):
+        plan = hx.Plan(m, layout=layout)
+        assert plan.layout() == layout
+        plan.set_state(s0)
+        if layout == "hermitian":
+            assert np.array_equal(plan.get_state(), s0)
+        r = plan.rhs()
+        plan.set_state(s0)
+        plan.step(steps)
+        out[layout] = (r, plan.get_state())
+        plan.close()
+    assert _rel(out["hermitian"][0], out["full"][0]) <= FP64_TOL
+    assert _rel(out["hermitian"][1], out["full"][1]) <= FP64_TOL
 
 
 @pytest.mark.parametrize("precision", ["fp32", "split_fp32"])
