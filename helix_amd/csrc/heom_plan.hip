@@ -55,6 +55,26 @@ struct C2t<float> {
 	using type = float2;
 };
 
+// Streaming accesses of the Taylor epilogue: a stage's output and the single-use Taylor terms are
+// neither re-read within the stage nor by a neighbour, so they are written and read non-temporally and
+// leave the L2 and the memory-side cache to the blocks other workgroups still gather as neighbours.
+template <typename T>
+__device__ __forceinline__ void st_stream(typename C2t<T>::type* p, T re, T im)
+{
+	typedef T ev2 __attribute__((ext_vector_type(2)));
+	ev2 t;
+	t.x = re;
+	t.y = im;
+	__builtin_nontemporal_store(t, reinterpret_cast<ev2*>(p));
+}
+template <typename T>
+__device__ __forceinline__ typename C2t<T>::type ld_stream(const typename C2t<T>::type* p)
+{
+	typedef T ev2 __attribute__((ext_vector_type(2)));
+	const ev2 t = __builtin_nontemporal_load(reinterpret_cast<const ev2*>(p));
+	return typename C2t<T>::type{t.x, t.y};
+}
+
 // Hermitian layout (include/helix_hip.h): stored index of full element (i, j) of an N x N block, and
 // whether the stored value must be conjugated.  The stored elements are rho[i][(i + d) mod N] for d < N/2
 // (the main part) and rho[t][t + N/2] for t < N/2 (the tail, after the main part); every other element
@@ -316,13 +336,13 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 		const T tr = ar * s, ti = ai * s;
 		V2* __restrict__ out = static_cast<V2*>(a.out);
 		if constexpr(!LAST) {
-			out[g] = V2{tr, ti};
+			st_stream<T>(out + g, tr, ti);
 		} else {
 			if(a.rho0_f64 != nullptr && n == 0) {
 				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
 				double2 r = r0[e];
 				for(int q = 0; q < a.ntprev; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					r.x += static_cast<double>(t.x);
 					r.y += static_cast<double>(t.y);
 				}
@@ -331,15 +351,15 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 				r0[e] = r;
 				out[g] = V2{static_cast<T>(r.x), static_cast<T>(r.y)};
 			} else {
-				V2 r = static_cast<const V2*>(a.rho)[g];
+				V2 r = ld_stream<T>(static_cast<const V2*>(a.rho) + g);
 				for(int q = 0; q < a.ntprev; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					r.x += t.x;
 					r.y += t.y;
 				}
 				r.x += tr;
 				r.y += ti;
-				out[g] = r;
+				st_stream<T>(out + g, r.x, r.y);
 			}
 		}
 	}
@@ -755,14 +775,14 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = static_cast<size_t>(n) * PE + qe;
 		if constexpr(!LAST) {
-			out[g] = V2{tr, ti};
+			st_stream<T>(out + g, tr, ti);
 		} else {
 			const int nt = a.ntprev;
 			if(a.rho0_f64 != nullptr && n == 0) {
 				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
 				double2 rv = r0[qe];
 				for(int q = 0; q + 1 < nt; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					rv.x += static_cast<double>(t.x);
 					rv.y += static_cast<double>(t.y);
 				}
@@ -775,9 +795,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				r0[qe] = rv;
 				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
 			} else {
-				V2 rv = (nt == 0) ? c : static_cast<const V2*>(a.rho)[g];
+				V2 rv = (nt == 0) ? c : ld_stream<T>(static_cast<const V2*>(a.rho) + g);
 				for(int q = 0; q + 1 < nt; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					rv.x += t.x;
 					rv.y += t.y;
 				}
@@ -787,7 +807,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				}
 				rv.x += tr;
 				rv.y += ti;
-				out[g] = rv;
+				st_stream<T>(out + g, rv.x, rv.y);
 			}
 		}
 	}
@@ -1073,13 +1093,13 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
 		if constexpr(!LAST) {
-			out[g] = V2{tr, ti};
+			st_stream<T>(out + g, tr, ti);
 		} else {
 			if(a.rho0_f64 != nullptr && n == 0) {
 				double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
 				double2 rv = r0[e];
 				for(int q = 0; q < a.ntprev; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					rv.x += static_cast<double>(t.x);
 					rv.y += static_cast<double>(t.y);
 				}
@@ -1088,15 +1108,15 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 				r0[e] = rv;
 				out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
 			} else {
-				V2 rv = static_cast<const V2*>(a.rho)[g];
+				V2 rv = ld_stream<T>(static_cast<const V2*>(a.rho) + g);
 				for(int q = 0; q < a.ntprev; ++q) {
-					const V2 t = static_cast<const V2*>(a.tprev[q])[g];
+					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
 					rv.x += t.x;
 					rv.y += t.y;
 				}
 				rv.x += tr;
 				rv.y += ti;
-				out[g] = rv;
+				st_stream<T>(out + g, rv.x, rv.y);
 			}
 		}
 	}
