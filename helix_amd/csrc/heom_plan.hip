@@ -450,7 +450,60 @@ struct TileLds {
 	int* mb;        // [kTileMaxModes] per mode: the bath (general couplings) or its sigma^z bit mask (SZ)
 	double* ka;     // [kTileMaxModes] per mode, SZ: 2 alpha of its bath
 	double* hd;     // [N] diagonal of H
+	void* wt;       // [2 kTileMaxModes][4] PatternWeight, sigma^z tiles of one ADO: each neighbour's weights
+	int* spread;    // [N] sigma^z tiles of one ADO: bit s of x moved to bit 2s
 };
+
+// bit s of x (x < 1024) moved to bit 2s
+__device__ __forceinline__ int spread_bits(int x)
+{
+	x = (x | (x << 8)) & 0x00FF00FF;
+	x = (x | (x << 4)) & 0x0F0F0F0F;
+	x = (x | (x << 2)) & 0x33333333;
+	x = (x | (x << 1)) & 0x55555555;
+	return x;
+}
+
+// Complex neighbour weight of one sigma^z bit pattern in the form the FMAs consume: FP64 or FP32 as (re, im),
+// split-FP32 as (hi re, hi im, lo re, lo im).
+template <typename T, bool SPLIT>
+struct PatternWeight {
+	using type = typename C2t<T>::type;
+	__device__ __forceinline__ static type make(double wr, double wi) { return type{static_cast<T>(wr), static_cast<T>(wi)}; }
+};
+template <>
+struct PatternWeight<float, true> {
+	using type = float4;
+	__device__ __forceinline__ static float4 make(double wr, double wi)
+	{
+		const float rh = static_cast<float>(wr), ih = static_cast<float>(wi);
+		return float4{rh, ih, static_cast<float>(wr - static_cast<double>(rh)), static_cast<float>(wi - static_cast<double>(ih))};
+	}
+};
+
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// acc += w * x with a prepared complex weight: four FMAs per term (split: the FP32 remainder first)
+template <typename T>
+__device__ __forceinline__ void pmadd(T& ar, T& ai, const typename C2t<T>::type& w, T xr, T xi)
+{
+	ar = fmx(w.x, xr, ar);
+	ai = fmx(w.x, xi, ai);
+	ar = fmx(-w.y, xi, ar);
+	ai = fmx(w.y, xr, ai);
+}
+__device__ __forceinline__ void pmadd(float& ar, float& ai, const float4& w, float xr, float xi)
+{
+	ar = __builtin_fmaf(w.z, xr, ar);
+	ai = __builtin_fmaf(w.z, xi, ai);
+	ar = __builtin_fmaf(-w.w, xi, ar);
+	ai = __builtin_fmaf(w.w, xr, ai);
+	ar = __builtin_fmaf(w.x, xr, ar);
+	ai = __builtin_fmaf(w.x, xi, ai);
+	ar = __builtin_fmaf(-w.y, xi, ar);
+	ai = __builtin_fmaf(w.y, xr, ai);
+}
 
 // Register slot r of a tile thread (lt = lane within its ADO): the stored element q within the ADO and
 // its full-block row i and column j.  Full layout: q = ebase + lt + TA r = i N + j.  Hermitian layout:
@@ -505,6 +558,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	constexpr int EPT = (HERM ? TE / 2 : TE) / TA;    // main elements per thread
 	constexpr int RT = HERM ? EPT + 1 : EPT;          // register slots per thread (+ the tail element)
 	constexpr size_t PE = HERM ? (static_cast<size_t>(E) + N) / 2 : static_cast<size_t>(E);  // stored per ADO
+	// sigma^z tiles of one ADO: each neighbour's weights for the four bit patterns of an element, tabled in LDS
+	constexpr bool TABLE = SZ && A == 1;
+	constexpr bool SPREAD_LDS = LOGN <= 8;  // N >= 512: computed, the LDS of two complex128 workgroups is full
+	using PW = PatternWeight<T, SPLIT>;
+	using WT = typename PW::type;
 	static_assert(!HERM || TA >= (TPA == 1 ? N / 2 : R), "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
@@ -514,6 +572,8 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	int* __restrict__ s_mb = lds.mb;
 	double* __restrict__ s_ka = lds.ka;
 	double* __restrict__ s_hd = lds.hd;
+	WT* __restrict__ s_wt = static_cast<WT*>(lds.wt);
+	int* __restrict__ s_spread = lds.spread;
 
 	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
@@ -542,25 +602,102 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		}
 	}
 	for(int q = tid; q < N; q += THREADS) s_hd[q] = a.hdiag[q];
+	if constexpr(TABLE && SPREAD_LDS)
+		for(int q = tid; q < N; q += THREADS) s_spread[q] = spread_bits(q);
 	// compacted neighbour lists: lanes 32s' .. 32s'+31 of a wave hold the 2M <= 32 candidate slots q of one ADO;
 	// a ballot gives each valid neighbour its position, so the stage loop visits valid entries only (the
 	// deepest tier, most of the hierarchy, has no up-neighbours)
 	static_assert(2 * kTileMaxModes == 32 && THREADS % 64 == 0, "one half-wave per ADO slot list");
-	for(int base = 0; base < A * 32; base += THREADS) {
-		const int t = base + tid;
-		const int s = t >> 5, k = t & 31;
-		const int nn = (group / TPA) * A + s;
-		int idx = -1;
-		if(s < A && k < 2 * M && nn < H) {
-			const int v = (k < M) ? a.up[nn * M + k] : a.down[nn * M + (k - M)];
-			idx = (v < H) ? v : -1;
-		}
-		const unsigned long long bal = __ballot(idx >= 0);
+	// neighbours are walked NB entries at a time: every load of a batch is issued before any arithmetic, so a
+	// thread waits for one memory round trip per NB neighbours (sized to what each variant's register budget
+	// holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
+	constexpr int NB = TABLE ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : 4)
+	                         : HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
+	                                : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
+	unsigned rem = 0;  // TABLE: the wave's valid candidate lanes not yet visited
+	int cand = -1;     // TABLE: lane k < 2M holds candidate slot k as (neighbour index << 5) | 2 s
+	int nvalid = 0;
+	if constexpr(TABLE) {
+		// every wave reads the ADO's 2M candidate slots (up n+e_k, then down n-e_k) and keeps the valid ones as a
+		// lane mask, so it knows its neighbours without waiting for the workgroup; entry (index << 5 | 2 s) has s
+		// the spin of the mode's bath.  Wave 0 tables the weights of each bit pattern p = 2 bit_s(i) + bit_s(j) of
+		// an element (i, j) by the entry's position: with v = alpha (2 bit - 1) the pair (v_i - v_j, v_i + v_j) is
+		// (0, -2a), (-2a, 0), (2a, 0), (0, 2a) and the weight is wc (v_i - v_j) + wa (v_i + v_j), wc = -i (up) or
+		// down_comm, wa = 0 (up) or down_anti.
 		const int lane = tid & 63;
-		const unsigned long long half = (lane < 32) ? 0xffffffffull : 0xffffffff00000000ull;
-		if(idx >= 0) s_nb[s][__popcll(bal & half & ((1ull << lane) - 1ull))] = (idx << 5) | k;
-		if(k == 0 && s < A) s_cnt[s] = __popcll(bal & half);
+		const int km = (lane < M) ? lane : lane - M;
+		int b = 0;
+		if(lane < 2 * M && live) {
+			const int v = (lane < M) ? a.up[n * M + km] : a.down[n * M + km];
+			b = a.mode_bath[km];
+			if(v < H) cand = (v << 5) | (2 * __builtin_ctz(static_cast<unsigned>(a.sz_mask[b])));
+		}
+		rem = static_cast<unsigned>(__ballot(cand >= 0));
+		nvalid = __popc(rem);
+		if(tid < 2 * M && cand >= 0) {
+			const int pos = __popc(rem & ((1u << lane) - 1u));
+			double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
+			if(lane >= M) {
+				wc = a.dcomm[n * M + km];
+				wa = a.danti[n * M + km];
+			}
+			const double ta = 2.0 * a.sz_alpha[b];
+			const double dd[4] = {0.0, -ta, ta, 0.0}, ss[4] = {-ta, 0.0, 0.0, ta};
+#pragma unroll
+			for(int pt = 0; pt < 4; ++pt)
+				s_wt[pos * 4 + pt] = PW::make(wc.x * dd[pt] + wa.x * ss[pt], wc.y * dd[pt] + wa.y * ss[pt]);
+		}
+	} else {
+		for(int base = 0; base < A * 32; base += THREADS) {
+			const int t = base + tid;
+			const int s = t >> 5, k = t & 31;
+			const int nn = (group / TPA) * A + s;
+			int idx = -1;
+			if(s < A && k < 2 * M && nn < H) {
+				const int v = (k < M) ? a.up[nn * M + k] : a.down[nn * M + (k - M)];
+				idx = (v < H) ? v : -1;
+			}
+			const unsigned long long bal = __ballot(idx >= 0);
+			const int lane = tid & 63;
+			const unsigned long long half = (lane < 32) ? 0xffffffffull : 0xffffffff00000000ull;
+			if(idx >= 0) s_nb[s][__popcll(bal & half & ((1ull << lane) - 1ull))] = (idx << 5) | k;
+			if(k == 0 && s < A) s_cnt[s] = __popcll(bal & half);
+		}
 	}
+	// TABLE: the first batch of neighbour elements is loaded before the tile is staged, so its round trip
+	// overlaps the staging, the barrier and -i[H, rho_n]; later batches follow in the neighbour loop
+	// (neighbour loads: a wave-uniform block base plus each element's 32-bit byte offset)
+	unsigned qb[RT];
+	int shb[NB];
+	V2 xb[NB][RT];
+#pragma unroll
+	for(int r = 0; r < RT; ++r) {
+		qb[r] = 0;
+		if(!HX_SLOT_LIVE(r)) continue;
+		int qe, i, j;
+		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+		qb[r] = static_cast<unsigned>(qe) * sizeof(V2);
+	}
+	auto load_batch = [&]() {
+#pragma unroll
+		for(int u = 0; u < NB; ++u) {
+#pragma unroll
+			for(int r = 0; r < RT; ++r) xb[u][r] = V2{T(0), T(0)};
+			shb[u] = 0;
+			if(rem == 0) continue;
+			const int pos = __builtin_ctz(rem);
+			rem &= rem - 1u;
+			const int packed = __builtin_amdgcn_readlane(cand, pos);
+			shb[u] = packed & 31;
+			const char* __restrict__ src = reinterpret_cast<const char*>(cur + static_cast<size_t>(packed >> 5) * PE);
+#pragma unroll
+			for(int r = 0; r < RT; ++r) {
+				if(!HX_SLOT_LIVE(r)) continue;
+				xb[u][r] = *reinterpret_cast<const V2*>(src + qb[r]);
+			}
+		}
+	};
+	if constexpr(TABLE) load_batch();
 	if constexpr(HERM && TPA > 1) {
 		// row band: pass 1 stores the band's own elements (one contiguous slice); pass 2 fills the rest of its
 		// rows with conjugates of elements stored in other rows, lanes walking the band rows at a fixed
@@ -656,81 +793,103 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		ai[r] = -hr[r];
 	}
 
-	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}},
-	// walked NB entries at a time: every load of a batch (neighbour elements and coefficients) is issued
-	// before any arithmetic, so a thread waits for one memory round trip per NB neighbours -- small
-	// problems, with few workgroups in flight, are bound by that chain.
-	// (sized to what each variant's register budget holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
-	constexpr int NB = HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
-	                        : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
-	const int nvalid = s_cnt[slot];
-	for(int c0 = 0; c0 < nvalid; c0 += NB) {
-		int nid[NB], nq[NB];
-		double2 wc[NB], wa[NB];
-		V2 x[NB][RT];
+	// neighbours: (-i)[V_b, rho_{n+e_k}] and down_comm [V_b, rho_{n-e_k}] + down_anti {V_b, rho_{n-e_k}}
+	if constexpr(TABLE) {
+		// each element's pattern word (bits 2s+1, 2s = bit s of i, bit s of j); an entry holds 2s, so its weight
+		// for the element is a two-bit field of the word and one LDS read
+		unsigned pw[RT];
 #pragma unroll
-		for(int u = 0; u < NB; ++u) {
-			const int packed = (c0 + u < nvalid) ? s_nb[slot][c0 + u] : -1;
-			nid[u] = packed >> 5;  // -1 when past the list
-			nq[u] = packed & 31;
-			const int q = nq[u];
-			wc[u] = make_double2(0.0, -1.0);  // up: weight -i on the commutator
-			wa[u] = make_double2(0.0, 0.0);
-#pragma unroll
-			for(int r = 0; r < RT; ++r) x[u][r] = V2{T(0), T(0)};
-			if(nid[u] < 0) continue;
-			if(q >= M) {
-				wc[u] = a.dcomm[n * M + (q - M)];
-				wa[u] = a.danti[n * M + (q - M)];
-			}
-			const V2* __restrict__ src = cur + static_cast<size_t>(nid[u]) * PE;
-#pragma unroll
-			for(int r = 0; r < RT; ++r) {
-				if(!HX_SLOT_LIVE(r)) continue;
-				int qe, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-				x[u][r] = src[qe];
-			}
+		for(int r = 0; r < RT; ++r) {
+			pw[r] = 0;
+			if(!HX_SLOT_LIVE(r)) continue;
+			int qe, i, j;
+			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+			pw[r] = SPREAD_LDS ? (s_spread[i] << 1) | s_spread[j] : (spread_bits(i) << 1) | spread_bits(j);
 		}
+		for(int c0 = 0;; c0 += NB) {
 #pragma unroll
-		for(int u = 0; u < NB; ++u) {
-			if(nid[u] < 0) continue;
-			const int q = nq[u];
-			const int k = q >= M ? q - M : q;
-			if constexpr(SZ) {
-				// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
-				// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
-				// Up: (-i) d x.  Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
-				// The two weights 2 alpha wc.y and 2 alpha wa.x are formed once; bit i only flips the sign.
-				const int mask = s_mb[k];
-				const double two_alpha = s_ka[k];
-				const Weight<T, SPLIT> wd = Weight<T, SPLIT>::make(two_alpha * wc[u].y);
-				const Weight<T, SPLIT> ws = Weight<T, SPLIT>::make(two_alpha * wa[u].x);
+			for(int u = 0; u < NB; ++u) {
+				if(c0 + u >= nvalid) continue;
+				const WT* __restrict__ w4 = s_wt + (c0 + u) * 4;
 #pragma unroll
 				for(int r = 0; r < RT; ++r) {
 					if(!HX_SLOT_LIVE(r)) continue;
-					int qe, i, j;
-					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-					const int bi = i & mask, bj = j & mask;
-					const bool differ = bi != bj;
-					const Weight<T, SPLIT> w = ws.select(differ, wd, bi == 0);
-					// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
-					const T re = differ ? -x[u][r].y : x[u][r].x;
-					const T im = differ ? x[u][r].x : x[u][r].y;
-					wmadd<T, SPLIT>(ar[r], ai[r], w, re, im);
+					pmadd(ar[r], ai[r], w4[(pw[r] >> shb[u]) & 3u], xb[u][r].x, xb[u][r].y);
 				}
-			} else {
-				const int b = s_mb[k];
-				const double* __restrict__ vb = s_v + b * N;
+			}
+			if(c0 + NB >= nvalid) break;
+			load_batch();
+		}
+	} else {
+		nvalid = s_cnt[slot];
+		for(int c0 = 0; c0 < nvalid; c0 += NB) {
+			int nid[NB], nq[NB];
+			double2 wc[NB], wa[NB];
+			V2 x[NB][RT];
+#pragma unroll
+			for(int u = 0; u < NB; ++u) {
+				const int packed = (c0 + u < nvalid) ? s_nb[slot][c0 + u] : -1;
+				nid[u] = packed >> 5;  // -1 when past the list
+				nq[u] = packed & 31;
+				const int q = nq[u];
+				wc[u] = make_double2(0.0, -1.0);  // up: weight -i on the commutator
+				wa[u] = make_double2(0.0, 0.0);
+#pragma unroll
+				for(int r = 0; r < RT; ++r) x[u][r] = V2{T(0), T(0)};
+				if(nid[u] < 0) continue;
+				if(q >= M) {
+					wc[u] = a.dcomm[n * M + (q - M)];
+					wa[u] = a.danti[n * M + (q - M)];
+				}
+				const V2* __restrict__ src = cur + static_cast<size_t>(nid[u]) * PE;
 #pragma unroll
 				for(int r = 0; r < RT; ++r) {
 					if(!HX_SLOT_LIVE(r)) continue;
 					int qe, i, j;
 					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
-					const double vi = vb[i], vj = vb[j];
-					const double d = vi - vj, sm = vi + vj;
-					cmadd<T, SPLIT>(ar[r], ai[r], wc[u].x * d + wa[u].x * sm, wc[u].y * d + wa[u].y * sm, x[u][r].x,
-						x[u][r].y);
+					x[u][r] = src[qe];
+				}
+			}
+#pragma unroll
+			for(int u = 0; u < NB; ++u) {
+				if(nid[u] < 0) continue;
+				const int q = nq[u];
+				const int k = q >= M ? q - M : q;
+				if constexpr(SZ) {
+					// d = v_i - v_j and s = v_i + v_j are +-2 alpha or 0, exactly one of them non-zero; with
+					// sg = +-2 alpha from bit i: bits differ -> d = sg, s = 0; bits equal -> d = 0, s = sg.
+					// Up: (-i) d x.  Down: i wc.y d x + wa.x s x (down_comm imaginary, down_anti real).
+					// The two weights 2 alpha wc.y and 2 alpha wa.x are formed once; bit i only flips the sign.
+					const int mask = s_mb[k];
+					const double two_alpha = s_ka[k];
+					const Weight<T, SPLIT> wd = Weight<T, SPLIT>::make(two_alpha * wc[u].y);
+					const Weight<T, SPLIT> ws = Weight<T, SPLIT>::make(two_alpha * wa[u].x);
+#pragma unroll
+					for(int r = 0; r < RT; ++r) {
+						if(!HX_SLOT_LIVE(r)) continue;
+						int qe, i, j;
+						tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+						const int bi = i & mask, bj = j & mask;
+						const bool differ = bi != bj;
+						const Weight<T, SPLIT> w = ws.select(differ, wd, bi == 0);
+						// commutator part multiplies by i: (x.x, x.y) -> (-x.y, x.x)
+						const T re = differ ? -x[u][r].y : x[u][r].x;
+						const T im = differ ? x[u][r].x : x[u][r].y;
+						wmadd<T, SPLIT>(ar[r], ai[r], w, re, im);
+					}
+				} else {
+					const int b = s_mb[k];
+					const double* __restrict__ vb = s_v + b * N;
+#pragma unroll
+					for(int r = 0; r < RT; ++r) {
+						if(!HX_SLOT_LIVE(r)) continue;
+						int qe, i, j;
+						tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+						const double vi = vb[i], vj = vb[j];
+						const double d = vi - vj, sm = vi + vj;
+						cmadd<T, SPLIT>(ar[r], ai[r], wc[u].x * d + wa[u].x * sm, wc[u].y * d + wa[u].y * sm, x[u][r].x,
+							x[u][r].y);
+					}
 				}
 			}
 		}
@@ -839,12 +998,15 @@ k_stage_tile(const StageArgs a, int ngroups)
 	__shared__ int s_mb[kTileMaxModes];
 	__shared__ double s_ka[kTileMaxModes];
 	__shared__ double s_hd[N];
+	constexpr bool TABLE = SZ && A == 1;
+	__shared__ typename PatternWeight<T, SPLIT>::type s_wt[TABLE ? 2 * kTileMaxModes * 4 : 1];
+	__shared__ int s_spread[(TABLE && LOGN <= 8) ? N : 1];
 	// XCD-aware renumbering: dispatch slot w runs on XCD w % 8
 	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
 	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(group >= ngroups) return;
 	tile_body<T, SPLIT, LAST, LOGN, THREADS, SZ, HERM>(a, group,
-		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd});
+		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread});
 }
 
 // Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
@@ -866,12 +1028,14 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	__shared__ int s_mb[kTileMaxModes];
 	__shared__ double s_ka[kTileMaxModes];
 	__shared__ double s_hd[1 << kBatchMaxLogN];
+	__shared__ typename PatternWeight<T, SPLIT>::type s_wt[2 * kTileMaxModes * 4];  // LOGN = 6 instances
+	__shared__ int s_spread[1 << kBatchMaxLogN];
 	const int per_xcd = (ntasks + kXcds - 1) / kXcds;
 	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
 	const StageArgs& a = table[tk.x];
-	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd};
+	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread};
 	switch(a.logN) {
 	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
 	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
