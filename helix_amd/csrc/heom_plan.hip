@@ -586,6 +586,18 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// the tail slot holds an element for lt < N/2 (whole blocks), or for lt < R in row tiles of the upper half
 	const bool tail_ok = HERM && (TPA == 1 ? lt < N / 2 : (lt < R && ebase < E / 2));
 #define HX_SLOT_LIVE(r) (!HERM || (r) < EPT || tail_ok)
+	// whole-block Hermitian tiles: the tile's own elements are loaded first, ahead of every table read
+	V2 ownv[RT];
+	if constexpr(HERM && TPA == 1) {
+#pragma unroll
+		for(int r = 0; r < RT; ++r) {
+			ownv[r] = V2{T(0), T(0)};
+			if(!live || !HX_SLOT_LIVE(r)) continue;
+			int q, i, j;
+			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+			ownv[r] = cur[static_cast<size_t>(n) * PE + q];
+		}
+	}
 
 	if constexpr(SZ) {
 		for(int q = tid; q < N; q += THREADS) s_selfx[q] = a.selfx[q];
@@ -626,11 +638,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		// down_comm, wa = 0 (up) or down_anti.
 		const int lane = tid & 63;
 		const int km = (lane < M) ? lane : lane - M;
-		int b = 0;
 		if(lane < 2 * M && live) {
 			const int v = (lane < M) ? a.up[n * M + km] : a.down[n * M + km];
-			b = a.mode_bath[km];
-			if(v < H) cand = (v << 5) | (2 * __builtin_ctz(static_cast<unsigned>(a.sz_mask[b])));
+			if(v < H) cand = (v << 5) | a.sz_shift[km];
 		}
 		rem = static_cast<unsigned>(__ballot(cand >= 0));
 		nvalid = __popc(rem);
@@ -641,7 +651,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				wc = a.dcomm[n * M + km];
 				wa = a.danti[n * M + km];
 			}
-			const double ta = 2.0 * a.sz_alpha[b];
+			const double ta = a.sz_ta[km];
 			const double dd[4] = {0.0, -ta, ta, 0.0}, ss[4] = {-ta, 0.0, 0.0, ta};
 #pragma unroll
 			for(int pt = 0; pt < 4; ++pt)
@@ -731,7 +741,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				if(!HX_SLOT_LIVE(r)) continue;
 				int q, i, j;
 				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
-				const V2 v = cur[static_cast<size_t>(n) * PE + q];
+				const V2 v = ownv[r];
 				s_blk[slot * TE + ((i << LOGN) | j)] = v;
 				if(i != j) s_blk[slot * TE + ((j << LOGN) | i)] = V2{v.x, -v.y};
 			}
@@ -1436,6 +1446,16 @@ int upload_csr(helix_hip_plan* p, const helix_hip_csr_operator* ops, int count, 
 	return rc;
 }
 
+// sigma^z couplings: per mode (the first 16), 2 s for the spin s of its bath and 2 alpha of the bath
+void set_mode_spins(StageArgs& a, const int32_t* mode_bath, int M)
+{
+	for(int k = 0; k < M && k < 16; ++k) {
+		const int b = mode_bath[k];
+		a.sz_shift[k] = 2 * __builtin_ctz(static_cast<unsigned>(a.sz_mask[b]));
+		a.sz_ta[k] = 2.0 * a.sz_alpha[b];
+	}
+}
+
 int ilog2_exact(int v)
 {
 	int l = 0;
@@ -2121,6 +2141,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	}
 	for(size_t q = 0; q < HM && a.sz; ++q)
 		if(pr->down_comm[2 * q] != 0.0 || pr->down_anti[2 * q + 1] != 0.0) a.sz = 0;
+	if(a.sz) set_mode_spins(a, pr->mode_bath, p->M);
 	if(a.sz) {
 		std::vector<double> selfx(static_cast<size_t>(p->N) * 2, 0.0);
 		for(int x = 0; x < p->N; ++x)
@@ -2468,6 +2489,7 @@ bool sigma_z_structure(const helix_hip_problem* pr, int logN, StageArgs& a, std:
 				selfx[2 * x] -= pr->xi[2 * b] * d2;
 				selfx[2 * x + 1] -= pr->xi[2 * b + 1] * d2;
 			}
+	set_mode_spins(a, pr->mode_bath, pr->n_modes);
 	a.sz = 1;
 	return true;
 }
