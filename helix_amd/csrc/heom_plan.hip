@@ -638,19 +638,19 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		// down_comm, wa = 0 (up) or down_anti.
 		const int lane = tid & 63;
 		const int km = (lane < M) ? lane : lane - M;
+		double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
 		if(lane < 2 * M && live) {
 			const int v = (lane < M) ? a.up[n * M + km] : a.down[n * M + km];
+			if(tid >= M && tid < 2 * M) {  // wave 0's down slots: coefficients loaded beside the index
+				wc = a.dcomm[n * M + km];
+				wa = a.danti[n * M + km];
+			}
 			if(v < H) cand = (v << 5) | a.sz_shift[km];
 		}
 		rem = static_cast<unsigned>(__ballot(cand >= 0));
 		nvalid = __popc(rem);
 		if(tid < 2 * M && cand >= 0) {
 			const int pos = __popc(rem & ((1u << lane) - 1u));
-			double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
-			if(lane >= M) {
-				wc = a.dcomm[n * M + km];
-				wa = a.danti[n * M + km];
-			}
 			const double ta = a.sz_ta[km];
 			const double dd[4] = {0.0, -ta, ta, 0.0}, ss[4] = {-ta, 0.0, 0.0, ta};
 #pragma unroll
