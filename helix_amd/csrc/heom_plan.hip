@@ -622,8 +622,8 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	static_assert(2 * kTileMaxModes == 32 && THREADS % 64 == 0, "one half-wave per ADO slot list");
 	// neighbours are walked NB entries at a time: every load of a batch is issued before any arithmetic, so a
 	// thread waits for one memory round trip per NB neighbours (sized to what each variant's register budget
-	// holds without spills: 128 VGPRs at 512 threads, 64 at 1024)
-	constexpr int NB = TABLE ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : 4)
+	// holds without spills: 128 VGPRs at 512 threads -- 64 for the complex64 N = 64 sigma^z tile --, 64 at 1024)
+	constexpr int NB = TABLE ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : (THREADS <= 512 ? 2 : 4))
 	                         : HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
 	                                : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
 	unsigned rem = 0;  // TABLE: the wave's valid candidate lanes not yet visited
@@ -984,16 +984,20 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 }
 
 // waves per SIMD the tile kernel is compiled for: two workgroups per CU, one when the complex128 tile and the
-// N = 1024 self table (80 KiB of LDS) leave room for only one
-template <typename T, int LOGN, int THREADS>
+// N = 1024 self table (80 KiB of LDS) leave room for only one.  The 512-thread complex64 Hermitian sigma^z tile of
+// one N = 64 ADO (36 KiB of LDS) is held to 64 VGPRs so that four workgroups -- four ADOs -- share a CU: its
+// stage is bound by each workgroup's memory round trips, and twice the ADOs in flight hide twice as many.
+template <typename T, int LOGN, int THREADS, bool SZ, bool HERM>
 constexpr int tile_waves()
 {
-	return (sizeof(T) == 8 && LOGN >= 10) ? THREADS / 256 : THREADS / 128;
+	return (sizeof(T) == 8 && LOGN >= 10) ? THREADS / 256
+	     : (sizeof(T) == 4 && THREADS == 512 && SZ && HERM && LOGN == 6) ? THREADS / 64
+	                                                                      : THREADS / 128;
 }
 
 template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, LOGN, THREADS>(),
-	tile_waves<T, LOGN, THREADS>()))) void
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, LOGN, THREADS, SZ, HERM>(),
+	tile_waves<T, LOGN, THREADS, SZ, HERM>()))) void
 k_stage_tile(const StageArgs a, int ngroups)
 {
 	using V2 = typename C2t<T>::type;
