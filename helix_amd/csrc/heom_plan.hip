@@ -987,6 +987,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 // N = 1024 self table (80 KiB of LDS) leave room for only one.  The 512-thread complex64 Hermitian sigma^z tile of
 // one N = 64 ADO (36 KiB of LDS) is held to 64 VGPRs so that four workgroups -- four ADOs -- share a CU: its
 // stage is bound by each workgroup's memory round trips, and twice the ADOs in flight hide twice as many.
+// (Row tiles of N = 128, 256 held the same way spill and run slower than their 1024-thread shape.)
 template <typename T, int LOGN, int THREADS, bool SZ, bool HERM>
 constexpr int tile_waves()
 {
@@ -1776,13 +1777,21 @@ int time_eager_steps(helix_hip_plan* p, int steps, double* ms)
 	return 0;
 }
 
-// Whole-block sigma^z tiles: time both workgroup shapes on the all-zero state (stepping zero leaves it
-// zero, and the kernels move the same bytes whatever the values) and keep the faster.
+// Whole-block sigma^z tiles (N <= 64) come in two workgroup shapes: time both
+// on the all-zero state (stepping zero leaves it zero, and the kernels move the same bytes whatever the values)
+// and keep the faster.  HELIX_HIP_TILE_THREADS=512 or 1024 forces one shape instead (diagnostics and tests).
 int autotune_threads(helix_hip_plan* p)
 {
 	const int def = (p->precision == HELIX_HIP_FP64) ? 512 : 1024;
 	p->threads = def;
-	if(!p->tiled || p->logN > 6 || !p->base.sz) return 0;
+	if(!p->tiled || !p->base.sz || p->logN > 6) return 0;
+	if(const char* force = std::getenv("HELIX_HIP_TILE_THREADS")) {
+		const int th = std::atoi(force);
+		if(th == 512 || th == 1024) {
+			p->threads = th;
+			return 0;
+		}
+	}
 	double best = 1e300;
 	int pick = def;
 	for(int th : {def, def == 512 ? 1024 : 512}) {

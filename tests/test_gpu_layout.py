@@ -83,6 +83,28 @@ def test_hermitian_row_tiles_equal_full_layout(hx, spins, L, steps):
     assert _rel(out["hermitian"][1], out["full"][1]) <= FP64_TOL
 
 
+@pytest.mark.parametrize("spins,L,precision,layout", [
+    (3, 4, "fp64", "hermitian"), (5, 2, "fp64", "hermitian"), (6, 2, "fp64", "hermitian"), (6, 2, "fp64", "full"),
+    (6, 2, "fp32", "hermitian"), (6, 2, "split_fp32", "hermitian"), (4, 3, "fp32", "hermitian"),
+    (6, 2, "fp32", "full")])
+def test_workgroup_shapes_agree_bitwise(hx, monkeypatch, spins, L, precision, layout):
+    """The stage kernel's two workgroup shapes for whole-block tiles (512 and 1024 threads, forced with
+    HELIX_HIP_TILE_THREADS; the plan's autotuner otherwise keeps the faster) compute every element with the
+    same operations in the same order, so their trajectories agree bit for bit."""
+    m = hx.Model.spin_glass(spins, 4, L=L, dt=0.01, lam=0.2, precision=precision)
+    s0 = _rand_state(m.H, m.N, 23) * 1e-2
+    out = {}
+    for th in ("512", "1024"):
+        monkeypatch.setenv("HELIX_HIP_TILE_THREADS", th)
+        plan = hx.Plan(m, layout=layout)
+        plan.set_state(s0)
+        plan.step(12)
+        out[th] = plan.get_state()
+        plan.close()
+    assert np.array_equal(out["512"], out["1024"]), _rel(out["512"], out["1024"])
+    assert np.all(np.isfinite(out["512"])) and np.max(np.abs(out["512"])) > 0
+
+
 @pytest.mark.parametrize("precision", ["fp32", "split_fp32"])
 def test_hermitian_layout_precision_forms(hx, oracle_mod, precision):
     m = hx.Model.spin_glass(3, 0, L=4, dt=0.01, precision=precision)
