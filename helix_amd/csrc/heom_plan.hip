@@ -1024,39 +1024,39 @@ k_stage_tile(const StageArgs a, int ngroups)
 		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread});
 }
 
-// Batched stage: one launch covers every (instance, tile) task of a batch of independent problems with
-// N <= 64 and sigma^z couplings; table[q] is instance q's argument block for this stage, tasks[t] =
-// (instance, tile group).  Tasks are numbered instance by instance, so the XCD renumbering keeps one
-// instance's tiles on one XCD.
-constexpr int kBatchMaxLogN = 6;
+// Batched stage: one launch per block size N covers every (instance, tile) task of the batch's instances of that
+// size (independent problems with N <= 64 and sigma^z couplings); table[q] is instance q's argument block for
+// this stage, tasks[t] = (instance, tile group).  Tasks are numbered instance by instance, so the XCD
+// renumbering keeps one instance's tiles on one XCD.  Specialising the launch by N sizes its registers and LDS
+// to that tile alone, so the complex64 N = 64 tile gets its four-per-CU shape here too.
+constexpr int kBatchMinLogN = 3;  // block sizes N = 8 .. 64: four groups
 
-template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
+template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM, int LOGN>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, LOGN, THREADS, true, HERM>(),
+	tile_waves<T, LOGN, THREADS, true, HERM>()))) void
 k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ tasks, int ntasks)
 {
 	using V2 = typename C2t<T>::type;
+	constexpr int N = 1 << LOGN;
+	constexpr int E = 1 << (2 * LOGN);
+	constexpr int A = (E >= kTile) ? 1 : kTile / E;
+	constexpr bool TABLE = A == 1;
 	__shared__ V2 s_blk[kTile];
 	__shared__ double s_v[1];
-	__shared__ double2 s_selfx[1 << kBatchMaxLogN];
-	__shared__ int s_nb[(kTile >> 6) * 2 * kTileMaxModes];
-	__shared__ int s_cnt[kTile >> 6];
+	__shared__ double2 s_selfx[N];
+	__shared__ int s_nb[A * 2 * kTileMaxModes];
+	__shared__ int s_cnt[A];
 	__shared__ int s_mb[kTileMaxModes];
 	__shared__ double s_ka[kTileMaxModes];
-	__shared__ double s_hd[1 << kBatchMaxLogN];
-	__shared__ typename PatternWeight<T, SPLIT>::type s_wt[2 * kTileMaxModes * 4];  // LOGN = 6 instances
-	__shared__ int s_spread[1 << kBatchMaxLogN];
+	__shared__ double s_hd[N];
+	__shared__ typename PatternWeight<T, SPLIT>::type s_wt[TABLE ? 2 * kTileMaxModes * 4 : 1];
+	__shared__ int s_spread[TABLE ? N : 1];
 	const int per_xcd = (ntasks + kXcds - 1) / kXcds;
 	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
-	const StageArgs& a = table[tk.x];
-	const TileLds<T> lds{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread};
-	switch(a.logN) {
-	case 3: tile_body<T, SPLIT, LAST, 3, THREADS, true, HERM>(a, tk.y, lds); break;
-	case 4: tile_body<T, SPLIT, LAST, 4, THREADS, true, HERM>(a, tk.y, lds); break;
-	case 5: tile_body<T, SPLIT, LAST, 5, THREADS, true, HERM>(a, tk.y, lds); break;
-	default: tile_body<T, SPLIT, LAST, 6, THREADS, true, HERM>(a, tk.y, lds); break;
-	}
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM>(table[tk.x], tk.y,
+		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread});
 }
 
 // Dense-Hamiltonian stage for N in {16, 32, 64}: one ADO per 256-thread workgroup.
@@ -2539,9 +2539,11 @@ struct helix_hip_batch {
 	void* t[kMaxOrder + 1] = {};
 	double* rho0_f64 = nullptr;
 	StageArgs* table = nullptr;  // [order][Q]
-	int2* tasks = nullptr;
-	int ntasks = 0;
-	int threads = 0;
+	int2* tasks = nullptr;  // grouped by block size: N = 2^(kBatchMinLogN + g) from task_off[g], task_cnt[g] long
+	int task_off[4] = {};
+	int task_cnt[4] = {};
+	int group_threads[4] = {};  // workgroup shape of each group (512 or 1024)
+	int threads = 0;            // shape of the group with the most tasks (reported by helix_hip_batch_info)
 	bool use_graph = false;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
@@ -2564,26 +2566,39 @@ int batch_alloc(helix_hip_batch* b, void** ptr, size_t bytes)
 	return 0;
 }
 
-template <typename T, bool SPLIT, int THREADS, bool HERM>
+template <typename T, bool SPLIT, int THREADS, bool HERM, int LOGN>
 void batch_launch_t(helix_hip_batch* b, int j)
 {
-	const unsigned grid = static_cast<unsigned>(((b->ntasks + kXcds - 1) / kXcds) * kXcds);
+	constexpr int g = LOGN - kBatchMinLogN;
+	const int n = b->task_cnt[g];
+	if(n == 0) return;
+	const unsigned grid = static_cast<unsigned>(((n + kXcds - 1) / kXcds) * kXcds);
 	const StageArgs* tab = b->table + static_cast<size_t>(j - 1) * b->Q;
+	const int2* tasks = b->tasks + b->task_off[g];
 	if(j == b->order)
-		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, true, THREADS, HERM>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
-			b->tasks, b->ntasks);
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, true, THREADS, HERM, LOGN>), dim3(grid), dim3(THREADS), 0, b->stream,
+			tab, tasks, n);
 	else
-		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, false, THREADS, HERM>), dim3(grid), dim3(THREADS), 0, b->stream, tab,
-			b->tasks, b->ntasks);
+		hipLaunchKernelGGL((k_batch_stage<T, SPLIT, false, THREADS, HERM, LOGN>), dim3(grid), dim3(THREADS), 0, b->stream,
+			tab, tasks, n);
+}
+
+template <typename T, bool SPLIT, bool HERM, int LOGN>
+void batch_launch_n(helix_hip_batch* b, int j)
+{
+	if(b->group_threads[LOGN - kBatchMinLogN] == 1024)
+		batch_launch_t<T, SPLIT, 1024, HERM, LOGN>(b, j);
+	else
+		batch_launch_t<T, SPLIT, 512, HERM, LOGN>(b, j);
 }
 
 template <typename T, bool SPLIT, bool HERM>
 void batch_launch_h(helix_hip_batch* b, int j)
 {
-	if(b->threads == 1024)
-		batch_launch_t<T, SPLIT, 1024, HERM>(b, j);
-	else
-		batch_launch_t<T, SPLIT, 512, HERM>(b, j);
+	batch_launch_n<T, SPLIT, HERM, 6>(b, j);
+	batch_launch_n<T, SPLIT, HERM, 5>(b, j);
+	batch_launch_n<T, SPLIT, HERM, 4>(b, j);
+	batch_launch_n<T, SPLIT, HERM, 3>(b, j);
 }
 
 template <typename T, bool SPLIT>
@@ -2749,10 +2764,17 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 		a.M = pr->n_modes;
 		a.B = pr->n_baths;
 		a.logN = b->logN[q];
-		const int E = 1 << (2 * a.logN);
-		const int A = (E >= kTile) ? 1 : kTile / E;
-		const int ngroups = (pr->n_ado + A - 1) / A;
-		for(int g = 0; g < ngroups; ++g) tasks.push_back(make_int2(q, g));
+	}
+	for(int g = 0; g < 4; ++g) {
+		b->task_off[g] = static_cast<int>(tasks.size());
+		for(int q = 0; q < count; ++q) {
+			if(b->logN[q] != kBatchMinLogN + g) continue;
+			const int E = 1 << (2 * b->logN[q]);
+			const int A = (E >= kTile) ? 1 : kTile / E;
+			const int ngroups = (problems[q]->n_ado + A - 1) / A;
+			for(int t = 0; t < ngroups; ++t) tasks.push_back(make_int2(q, t));
+		}
+		b->task_cnt[g] = static_cast<int>(tasks.size()) - b->task_off[g];
 	}
 	void* dar = nullptr;
 	if((rc = batch_alloc(b, &dar, arena.bytes.size()))) return bail(rc);
@@ -2798,28 +2820,38 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 	if((rc = batch_alloc(b, &dk, tasks.size() * sizeof(int2)))) return bail(rc);
 	HX_HIP(hipMemcpy(dk, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice));
 	b->tasks = static_cast<int2*>(dk);
-	b->ntasks = static_cast<int>(tasks.size());
 	HX_HIP(hipMemsetAsync(b->rho, 0, b->total_elems * b->esize, b->stream));
 	for(int q = 1; q < b->order; ++q) HX_HIP(hipMemsetAsync(b->t[q], 0, b->total_elems * b->esize, b->stream));
 	if(b->rho0_f64) HX_HIP(hipMemsetAsync(b->rho0_f64, 0, r0 * 16, b->stream));
 	HX_HIP(hipStreamSynchronize(b->stream));
 
-	// workgroup shape: complex128 tiles default to 512 threads, complex64 to 1024; the autotuner times
-	// both on the all-zero state (stepping zero leaves it zero) and keeps the faster
-	b->threads = (b->precision == HELIX_HIP_FP64) ? 512 : 1024;
+	// workgroup shape per block size: complex128 tiles default to 512 threads, complex64 to 1024; the autotuner
+	// times each group's two shapes on the all-zero state (stepping zero leaves it zero), the other groups
+	// fixed, and keeps the faster
+	const int def = (b->precision == HELIX_HIP_FP64) ? 512 : 1024;
+	for(int g = 0; g < 4; ++g) b->group_threads[g] = def;
 	if(flags & HELIX_HIP_BATCH_AUTOTUNE) {
-		double best = 1e300;
-		int pick = b->threads;
-		for(int th : {512, 1024}) {
-			b->threads = th;
-			double ms = 0.0;
-			if((rc = batch_time(b, 1, &ms)) || (rc = batch_time(b, 3, &ms))) return bail(rc);
-			if(ms < best) {
-				best = ms;
-				pick = th;
+		for(int g = 0; g < 4; ++g) {
+			if(b->task_cnt[g] == 0) continue;
+			double best = 1e300;
+			int pick = def;
+			for(int th : {512, 1024}) {
+				b->group_threads[g] = th;
+				double ms = 0.0;
+				if((rc = batch_time(b, 1, &ms)) || (rc = batch_time(b, 3, &ms))) return bail(rc);
+				if(ms < best) {
+					best = ms;
+					pick = th;
+				}
 			}
+			b->group_threads[g] = pick;
 		}
-		b->threads = pick;
+	}
+	{
+		int most = 0;
+		for(int g = 1; g < 4; ++g)
+			if(b->task_cnt[g] > b->task_cnt[most]) most = g;
+		b->threads = b->group_threads[most];
 	}
 	if(flags & HELIX_HIP_BATCH_GRAPH) {
 		HX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
