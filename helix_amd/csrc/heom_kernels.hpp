@@ -1,8 +1,10 @@
 // heom_kernels.hpp -- gfx950 kernels of the HEOM Liouville action and its Taylor-m step.
 //
-// Layout in HBM: a state vector is [H][N][N] complex, row-major blocks, ADOs in the hierarchy's
-// lexicographic order (src/initialize_detail.h:174-214).  No sentinel block is stored: absent
-// neighbours are skipped by index (the reference reads a zero block, src/initialize.cu:289,306).
+// Layout in HBM: a state vector is H blocks of complex, one per ADO: N x N row-major (full layout) or the
+// N(N+1)/2 stored elements of the Hermitian layout (include/helix_hip.h).  On the device the ADOs are stored
+// in colexicographic order of their multi-indices; the C-ABI converts from and to the hierarchy's
+// lexicographic order (src/initialize_detail.h:174-214) at the boundary.  No sentinel block is stored:
+// absent neighbours are skipped by index (the reference reads a zero block, src/initialize.cu:289,306).
 // A Taylor stage j of order m is one fused launch (diagonal couplings) or two launches (couplings
 // with bit flips): it evaluates L(cur) and applies the recurrence in its epilogue,
 //   j <  m : t_j = (dt/j) L(t_{j-1})                          (t_0 = rho)

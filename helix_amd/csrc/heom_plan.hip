@@ -368,18 +368,20 @@ __global__ __launch_bounds__(kBlock) void k_stage(const StageArgs a)
 // LDS-tiled stage for diagonal couplings, N <= 1024 (LOGN 3..10).
 // A workgroup owns a tile of 4096 elements = A = 4096/N^2 whole ADO blocks (N <= 64) or one row band of
 // one block (N > 64, TPA bands per block); each ADO is handled by TA = THREADS/A threads, element
-// e = lane_in_ado + TA r, so every global access is a contiguous 16-byte-per-lane row.  The tile's own
-// blocks are staged once in LDS, so every Hamiltonian bit-flip gather (rows i^m, columns j^m) is served
-// from LDS.  The per-ADO neighbour table (n+e_k, n-e_k) is copied to LDS and walked entry by entry; each
-// valid neighbour issues all of its loads before any arithmetic.  Workgroups are renumbered so each XCD
-// (round-robin dispatch over 8 XCDs) sweeps one contiguous range of ADOs: the near neighbours in
-// lexicographic order (n +- e_k for the trailing modes) then hit that XCD's L2.
+// e = lane_in_ado + TA r, so every global access is a contiguous row of lanes.  The tile's own blocks are
+// staged once in LDS, so every Hamiltonian bit-flip gather (rows i^m, columns j^m) is served from LDS.
+// Neighbours (n+e_k, n-e_k) are walked a batch at a time, every load of a batch issued before any
+// arithmetic: tiles of several ADOs keep a compacted list per ADO in LDS; sigma^z tiles of one ADO (TABLE)
+// build the list per wave from a ballot and table each neighbour's weights per bit pattern in LDS.
+// Workgroups are renumbered so each XCD (round-robin dispatch over 8 XCDs) sweeps one contiguous range of
+// the device's ADO order (colexicographic), so the near neighbours hit that XCD's L2.
 // SZ: every coupling is alpha_b (2 n_{s_b} - 1): commutator / anticommutator weights come from bits
 // (d_b = v_i - v_j and s_b = v_i + v_j, one of them zero) and the Sigma + Xi weight from selfx[i^j].
-// HERM: the Hermitian layout of include/helix_hip.h (N(N+1)/2 stored elements per ADO, N <= 64).  The
-// stored elements are loaded once and written to both (i, j) and (j, i) of the full block in LDS; the
-// tile computes only the stored elements (every term of L is elementwise in the neighbours, and the
-// Hamiltonian gathers come from the full LDS block), so loads, stores and neighbour reads all halve.
+// HERM: the Hermitian layout of include/helix_hip.h (N(N+1)/2 stored elements per ADO).  The stored elements
+// are loaded once and written to both (i, j) and (j, i) of the block in LDS (a row band completes its rows
+// from the transposed elements of the other rows); the tile computes only the stored elements (every term
+// of L is elementwise in the neighbours, and the Hamiltonian gathers come from the LDS block), so loads,
+// stores and neighbour reads all halve.
 constexpr int kTile = 4096;  // elements per workgroup
 constexpr int kTileMaxModes = 16;
 constexpr int kXcds = 8;
