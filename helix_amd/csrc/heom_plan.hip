@@ -76,27 +76,47 @@ __device__ __forceinline__ typename C2t<T>::type ld_stream(const typename C2t<T>
 }
 
 // Hermitian layout (include/helix_hip.h): stored index of full element (i, j) of an N x N block, and
-// whether the stored value must be conjugated.  The stored elements are rho[i][(i + d) mod N] for d < N/2
-// (the main part) and rho[t][t + N/2] for t < N/2 (the tail, after the main part); every other element
-// is the conjugate of a stored one.  The main part is ordered d-major, [N/2][N] with element (d, i), for
-// N <= 64 (a lane walk over i at fixed d is contiguous and a workgroup holds whole blocks), and i-major,
-// [N][N/2] with element (i, d), for N > 64 (a band of rows is one contiguous slice, as the row tiles read it).
-__host__ __device__ inline int herm_main(int i, int d, int logN)
+// whether the stored value must be conjugated.
+// N <= 64 (whole blocks in a workgroup): ordered by x = i XOR j.  Main part [N][N/2], element (x, k):
+// x = 0 is the diagonal rho[k][k]; for x > 0 it is rho[i][i ^ x] with i the k-th index whose bit h(x)
+// (the highest set bit of x) is clear, so i < i ^ x.  Tail [N/2]: rho[N/2 + t][N/2 + t].  A sigma^z
+// coupling of spin s has a zero commutator weight on every element whose x has bit s clear, and each row of
+// the main part has one x, so a commutator-only neighbour term needs only the rows with bit s set.
+// N > 64 (row bands): stored elements rho[i][(i + d) mod N] for d < N/2 as [N][N/2] element (i, d) (a band
+// of rows is one contiguous slice, as the row tiles read it) and the tail rho[t][t + N/2] for t < N/2.
+__host__ __device__ inline int herm_insert0(int k, int h)  // k with a zero bit inserted at position h
 {
-	return (logN <= 6) ? ((d << logN) | i) : ((i << (logN - 1)) | d);
+	return ((k >> h) << (h + 1)) | (k & ((1 << h) - 1));
+}
+
+__host__ __device__ inline int herm_high_bit(int x)  // x > 0
+{
+	return 31 - __builtin_clz(static_cast<unsigned>(x));
 }
 
 __host__ __device__ inline int herm_index(int i, int j, int logN, bool& conj)
 {
 	const int N = 1 << logN;
+	if(logN <= 6) {
+		const int x = i ^ j;
+		if(x == 0) {
+			conj = false;
+			return (i < N / 2) ? i : N * N / 2 + (i - N / 2);
+		}
+		const int lo = (i < j) ? i : j;
+		conj = i > j;
+		const int h = herm_high_bit(x);
+		const int k = ((lo >> (h + 1)) << h) | (lo & ((1 << h) - 1));
+		return x * (N / 2) + k;
+	}
 	const int d = (j - i) & (N - 1);
 	if(d < N / 2) {
 		conj = false;
-		return herm_main(i, d, logN);
+		return (i << (logN - 1)) | d;
 	}
 	if(d > N / 2) {
 		conj = true;
-		return herm_main(j, N - d, logN);
+		return (j << (logN - 1)) | (N - d);
 	}
 	conj = i >= N / 2;
 	return N * N / 2 + (conj ? j : i);
@@ -508,9 +528,10 @@ __device__ __forceinline__ void pmadd(float& ar, float& ai, const float4& w, flo
 }
 
 // Register slot r of a tile thread (lt = lane within its ADO): the stored element q within the ADO and
-// its full-block row i and column j.  Full layout: q = ebase + lt + TA r = i N + j.  Hermitian layout:
-// slots r < EPT are main elements q = lt + TA r = d N + i, j = (i + d) mod N; slot EPT is the tail
-// element q = N^2/2 + lt (i = lt, j = lt + N/2), held by threads lt < N/2.
+// its full-block row i and column j.  Full layout: q = ebase + lt + TA r = i N + j.  Hermitian layout,
+// whole blocks: slots r < EPT are main elements q = lt + TA r = x N/2 + k (row x = i XOR j, see herm_index);
+// slot EPT is the tail element q = N^2/2 + lt (the diagonal element i = j = N/2 + lt), held by threads
+// lt < N/2.  Row bands: the band's slice of the [N][N/2] main part and its rows' tail elements.
 template <int LOGN, int TA, int EPT, bool HERM>
 __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int& i, int& j)
 {
@@ -518,12 +539,13 @@ __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int&
 	if constexpr(HERM && LOGN <= 6) {
 		if(r < EPT) {
 			q = lt + TA * r;
-			i = q & (N - 1);
-			j = (i + (q >> LOGN)) & (N - 1);
+			const int x = q >> (LOGN - 1), k = q & (N / 2 - 1);
+			i = (x == 0) ? k : herm_insert0(k, herm_high_bit(x));
+			j = i ^ x;
 		} else {
 			q = N * N / 2 + lt;
-			i = lt;
-			j = lt + N / 2;
+			i = N / 2 + lt;
+			j = i;
 		}
 	} else if constexpr(HERM) {
 		// row band of R = 4096/N rows from r0 = ebase / N: its stored main part is one contiguous slice
@@ -643,11 +665,13 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
 		if(lane < 2 * M && live) {
 			const int v = (lane < M) ? a.up[n * M + km] : a.down[n * M + km];
-			if(tid >= M && tid < 2 * M) {  // wave 0's down slots: coefficients loaded beside the index
+			if(lane >= M) {  // down slots: coefficients loaded beside the index
 				wc = a.dcomm[n * M + km];
 				wa = a.danti[n * M + km];
 			}
-			if(v < H) cand = (v << 5) | a.sz_shift[km];
+			// entry: (index << 6) | (anticommutator weight present << 5) | 2 s
+			const int anti = (wa.x != 0.0 || wa.y != 0.0) ? 1 : 0;
+			if(v < H) cand = (v << 6) | (anti << 5) | a.sz_shift[km];
 		}
 		rem = static_cast<unsigned>(__ballot(cand >= 0));
 		nvalid = __popc(rem);
@@ -690,6 +714,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
 		qb[r] = static_cast<unsigned>(qe) * sizeof(V2);
 	}
+	// Whole Hermitian blocks (rows of one x = i XOR j): a neighbour without an anticommutator weight has a
+	// zero weight on every element whose x has bit s clear (bits s of i and j equal), so those elements -- half
+	// of each such block, whole rows -- are not loaded (the tail is the diagonal, x = 0).
+	constexpr bool SKIP = TABLE && HERM && TPA == 1;
+	constexpr int QSH = (sizeof(V2) == 16 ? 4 : 3) + LOGN - 1;  // byte offset of a main element >> QSH = x
 	auto load_batch = [&]() {
 #pragma unroll
 		for(int u = 0; u < NB; ++u) {
@@ -701,10 +730,13 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			rem &= rem - 1u;
 			const int packed = __builtin_amdgcn_readlane(cand, pos);
 			shb[u] = packed & 31;
-			const char* __restrict__ src = reinterpret_cast<const char*>(cur + static_cast<size_t>(packed >> 5) * PE);
+			const bool anti = (packed >> 5) & 1;
+			const char* __restrict__ src = reinterpret_cast<const char*>(cur + static_cast<size_t>(packed >> 6) * PE);
 #pragma unroll
 			for(int r = 0; r < RT; ++r) {
 				if(!HX_SLOT_LIVE(r)) continue;
+				if constexpr(SKIP)
+					if(!anti && (r == EPT || !((qb[r] >> (QSH + (shb[u] >> 1))) & 1u))) continue;
 				xb[u][r] = *reinterpret_cast<const V2*>(src + qb[r]);
 			}
 		}

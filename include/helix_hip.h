@@ -52,9 +52,12 @@ enum helix_hip_precision {
  * or Matsubara poles), down_comm is imaginary, down_anti real, the self and Xi coefficients real and
  * KXiCounter imaginary, the Hamiltonian and couplings Hermitian, then L maps Hermitian ADOs to
  * Hermitian ADOs and a Hermitian initial state keeps every ADO Hermitian.  The Hermitian layout
- * stores N(N+1)/2 elements per ADO instead of N^2 (about half the HBM traffic):
- *   main (N^2/2):  rho[i][(i + d) mod N] for 0 <= d < N/2, as [N/2][N] element (d, i) for N <= 64
- *                  and as [N][N/2] element (i, d) for N > 64
+ * stores N(N+1)/2 elements per ADO instead of N^2 (about half the HBM traffic).  N <= 64:
+ *   main [N][N/2]: element (x, k) = rho[k][k] for x = 0, else rho[i][i ^ x] with i the k-th index whose
+ *                  bit h is clear, h the highest set bit of x (so i < i ^ x)
+ *   tail [N/2]:    element t = rho[N/2 + t][N/2 + t]
+ * N > 64:
+ *   main [N][N/2]: element (i, d) = rho[i][(i + d) mod N] for 0 <= d < N/2
  *   tail [N/2]:    element t = rho[t][t + N/2]
  * every other element is the conjugate of a stored one.  States passed in must then be Hermitian
  * (to 1e-12 relative), else the call returns HELIX_HIP_EINVAL; states passed out are full blocks. */
