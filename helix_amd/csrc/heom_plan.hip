@@ -647,7 +647,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// neighbours are walked NB entries at a time: every load of a batch is issued before any arithmetic, so a
 	// thread waits for one memory round trip per NB neighbours (sized to what each variant's register budget
 	// holds without spills: 128 VGPRs at 512 threads -- 64 for the complex64 N = 64 sigma^z tile --, 64 at 1024)
-	constexpr int NB = TABLE ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : (THREADS <= 512 ? 2 : 4))
+	constexpr int NB = TABLE ? (sizeof(T) == 8 ? 2 : (THREADS <= 512 ? 2 : 4))
 	                         : HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
 	                                : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
 	unsigned rem = 0;  // TABLE: the wave's valid candidate lanes not yet visited
