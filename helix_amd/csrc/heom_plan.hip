@@ -628,21 +628,23 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	} else {
 		for(int q = tid; q < B * N; q += THREADS) s_v[q] = a.vdiag[q];
 	}
-	for(int q = tid; q < M; q += THREADS) {
-		const int b = a.mode_bath[q];
-		if constexpr(SZ) {
-			s_mb[q] = a.sz_mask[b];
-			s_ka[q] = 2.0 * a.sz_alpha[b];
-		} else {
-			s_mb[q] = b;
+	if constexpr(!TABLE) {
+		for(int q = tid; q < M; q += THREADS) {
+			const int b = a.mode_bath[q];
+			if constexpr(SZ) {
+				s_mb[q] = a.sz_mask[b];
+				s_ka[q] = 2.0 * a.sz_alpha[b];
+			} else {
+				s_mb[q] = b;
+			}
 		}
 	}
 	for(int q = tid; q < N; q += THREADS) s_hd[q] = a.hdiag[q];
 	if constexpr(TABLE && SPREAD_LDS)
 		for(int q = tid; q < N; q += THREADS) s_spread[q] = spread_bits(q);
-	// compacted neighbour lists: lanes 32s' .. 32s'+31 of a wave hold the 2M <= 32 candidate slots q of one ADO;
-	// a ballot gives each valid neighbour its position, so the stage loop visits valid entries only (the
-	// deepest tier, most of the hierarchy, has no up-neighbours)
+	// neighbour lists hold valid entries only (the deepest tier, most of the hierarchy, has no up-neighbours):
+	// a ballot over the 2M <= 32 candidate slots of an ADO, held by 32 lanes of a wave, gives each valid
+	// neighbour its position
 	static_assert(2 * kTileMaxModes == 32 && THREADS % 64 == 0, "one half-wave per ADO slot list");
 	// neighbours are walked NB entries at a time: every load of a batch is issued before any arithmetic, so a
 	// thread waits for one memory round trip per NB neighbours (sized to what each variant's register budget
@@ -651,12 +653,12 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	                         : HERM ? (sizeof(T) == 8 ? (THREADS <= 512 ? 2 : 1) : ((THREADS <= 512 || LOGN >= 6) ? 4 : 2))
 	                                : (sizeof(T) == 8 ? 1 : (THREADS <= 512 ? 1 : 2));
 	unsigned rem = 0;  // TABLE: the wave's valid candidate lanes not yet visited
-	int cand = -1;     // TABLE: lane k < 2M holds candidate slot k as (neighbour index << 5) | 2 s
+	int cand = -1;     // TABLE: lane k < 2M holds candidate slot k as an entry (see below)
 	int nvalid = 0;
 	if constexpr(TABLE) {
 		// every wave reads the ADO's 2M candidate slots (up n+e_k, then down n-e_k) and keeps the valid ones as a
-		// lane mask, so it knows its neighbours without waiting for the workgroup; entry (index << 5 | 2 s) has s
-		// the spin of the mode's bath.  Wave 0 tables the weights of each bit pattern p = 2 bit_s(i) + bit_s(j) of
+		// lane mask, so it knows its neighbours without waiting for the workgroup; an entry is
+		// (index << 6) | (anticommutator weight present << 5) | 2 s, s the spin of the mode's bath.  Wave 0 tables the weights of each bit pattern p = 2 bit_s(i) + bit_s(j) of
 		// an element (i, j) by the entry's position: with v = alpha (2 bit - 1) the pair (v_i - v_j, v_i + v_j) is
 		// (0, -2a), (-2a, 0), (2a, 0), (0, 2a) and the weight is wc (v_i - v_j) + wa (v_i + v_j), wc = -i (up) or
 		// down_comm, wa = 0 (up) or down_anti.
@@ -669,7 +671,6 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				wc = a.dcomm[n * M + km];
 				wa = a.danti[n * M + km];
 			}
-			// entry: (index << 6) | (anticommutator weight present << 5) | 2 s
 			const int anti = (wa.x != 0.0 || wa.y != 0.0) ? 1 : 0;
 			if(v < H) cand = (v << 6) | (anti << 5) | a.sz_shift[km];
 		}
