@@ -58,6 +58,7 @@ struct StageArgs {
 	double sz_alpha[16];
 	int sz_shift[16];         // per mode k < 16: 2 s for the spin s of its bath
 	double sz_ta[16];         // per mode k < 16: 2 alpha of its bath
+	int sz_anti;              // bit k (k < 16): some ADO has a non-zero down_anti for mode k
 	const double2* selfx;     // [N]
 	// dense Hamiltonian (MFMA commutator path): [N][N] complex in the state precision; for the
 	// split-FP32 form hdense holds the FP32 high part and hdense_lo the FP32 remainder

@@ -667,11 +667,11 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
 		if(lane < 2 * M && live) {
 			const int v = (lane < M) ? a.up[n * M + km] : a.down[n * M + km];
-			if(lane >= M) {  // down slots: coefficients loaded beside the index
+			if(tid >= M && tid < 2 * M) {  // wave 0's down slots: coefficients for the weight table
 				wc = a.dcomm[n * M + km];
 				wa = a.danti[n * M + km];
 			}
-			const int anti = (wa.x != 0.0 || wa.y != 0.0) ? 1 : 0;
+			const int anti = (lane >= M) ? ((a.sz_anti >> km) & 1) : 0;
 			if(v < H) cand = (v << 6) | (anti << 5) | a.sz_shift[km];
 		}
 		rem = static_cast<unsigned>(__ballot(cand >= 0));
@@ -1486,13 +1486,22 @@ int upload_csr(helix_hip_plan* p, const helix_hip_csr_operator* ops, int count, 
 	return rc;
 }
 
-// sigma^z couplings: per mode (the first 16), 2 s for the spin s of its bath and 2 alpha of the bath
-void set_mode_spins(StageArgs& a, const int32_t* mode_bath, int M)
+// sigma^z couplings: per mode (the first 16), 2 s for the spin s of its bath, 2 alpha of the bath, and whether
+// any ADO has an anticommutator (down_anti) weight for it
+void set_mode_spins(StageArgs& a, const helix_hip_problem* pr)
 {
-	for(int k = 0; k < M && k < 16; ++k) {
-		const int b = mode_bath[k];
+	a.sz_anti = 0;
+	for(int k = 0; k < pr->n_modes && k < 16; ++k) {
+		const int b = pr->mode_bath[k];
 		a.sz_shift[k] = 2 * __builtin_ctz(static_cast<unsigned>(a.sz_mask[b]));
 		a.sz_ta[k] = 2.0 * a.sz_alpha[b];
+		for(int n = 0; n < pr->n_ado; ++n) {
+			const double* w = pr->down_anti + 2 * (static_cast<size_t>(n) * pr->n_modes + k);
+			if(w[0] != 0.0 || w[1] != 0.0) {
+				a.sz_anti |= 1 << k;
+				break;
+			}
+		}
 	}
 }
 
@@ -2189,7 +2198,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	}
 	for(size_t q = 0; q < HM && a.sz; ++q)
 		if(pr->down_comm[2 * q] != 0.0 || pr->down_anti[2 * q + 1] != 0.0) a.sz = 0;
-	if(a.sz) set_mode_spins(a, pr->mode_bath, p->M);
+	if(a.sz) set_mode_spins(a, pr);
 	if(a.sz) {
 		std::vector<double> selfx(static_cast<size_t>(p->N) * 2, 0.0);
 		for(int x = 0; x < p->N; ++x)
@@ -2537,7 +2546,7 @@ bool sigma_z_structure(const helix_hip_problem* pr, int logN, StageArgs& a, std:
 				selfx[2 * x] -= pr->xi[2 * b] * d2;
 				selfx[2 * x + 1] -= pr->xi[2 * b + 1] * d2;
 			}
-	set_mode_spins(a, pr->mode_bath, pr->n_modes);
+	set_mode_spins(a, pr);
 	a.sz = 1;
 	return true;
 }
