@@ -20,7 +20,7 @@ def _models(hx, seeds, precision="fp64"):
     return [hx.Model.spin_glass_random(s, dt=0.01, precision=precision) for s in seeds]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "split_fp32"])
 def test_batch_equals_single_plans_bitwise(hx, precision):
     seeds = list(range(1000, 1012))
     models = _models(hx, seeds, precision)
