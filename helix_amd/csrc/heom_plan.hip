@@ -818,9 +818,12 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
 			const int e = (i << LOGN) | j;
 			const int le = e - ebase;
+			// LDS gathers by byte offset: (le ^ f) * size == (le * size) ^ (f * size), one XOR per gather
+			const char* __restrict__ bb = reinterpret_cast<const char*>(blk);
+			const unsigned lb = static_cast<unsigned>(le) * sizeof(V2);
 			V2 x1;
 			if(TPA == 1 || rf < kTile) {
-				x1 = blk[le ^ rf];
+				x1 = *reinterpret_cast<const V2*>(bb + (lb ^ (static_cast<unsigned>(rf) * sizeof(V2))));
 			} else if constexpr(HERM) {  // row i^m lies outside the band: its element from the packed block
 				bool cj;
 				x1 = ownh[herm_index(i ^ m, j, LOGN, cj)];
@@ -828,7 +831,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			} else {
 				x1 = own[e ^ rf];
 			}
-			const V2 x2 = blk[le ^ m];
+			const V2 x2 = *reinterpret_cast<const V2*>(bb + (lb ^ (static_cast<unsigned>(m) * sizeof(V2))));
 			wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x - x2.x, x1.y - x2.y);
 		}
 	}
