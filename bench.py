@@ -221,9 +221,6 @@ def bench_legacy(args):
 def bench_batched(args):
     """512 independent spin-glass instances (seeds 1000..1511: spins 3-6, tiers 2-5, random bath
     parameters, dt 0.01) advanced together, one launch per Taylor stage, step replayed from a hipGraph."""
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported"}))
-        return 2
     import helix_amd as hx
     models = [hx.Model.spin_glass_random(1000 + i, dt=0.01, precision=args.precision)
               for i in range(args.batch_count)]
@@ -276,16 +273,16 @@ def main(argv=None):
                     help="also run one device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args(argv)
 
+    if args.gpus != 1:  # every workload: the figure is one GPU's, and nothing here spans GPUs
+        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported",
+                          "n_gpus": args.gpus}))
+        return 2
     if args.workload.startswith("batched"):
         return bench_batched(args)
     if args.workload == "learned":
         return bench_learned(args)
     if args.workload == "legacy":
         return bench_legacy(args)
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported",
-                          "n_gpus": args.gpus}))
-        return 2
 
     import helix_amd as hx
 
