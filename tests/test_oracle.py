@@ -291,3 +291,112 @@ def test_spin_glass_couplings_fixture(oracle_mod):
     for key, entry in fx["couplings"].items():
         J = oracle_mod.tfim_couplings(entry["spins"], entry["seed"])
         assert np.array_equal(J, np.array(entry["J"]))
+
+
+# ---------------------------------------------- TFIM Hamiltonian, <H>, normal draws
+#
+# hx_tfim_hamiltonian builds H for every spin-glass workload and hx_trace_h_rho gives <H>(t); both are
+# pinned here against constructions that share nothing with them: Kronecker products of 2x2 Pauli
+# matrices, numpy's trace of a dense product, and spectra known in closed form.
+
+_X = np.array([[0.0, 1.0], [1.0, 0.0]])
+_Z_BIT = np.diag([-1.0, 1.0])  # basis bit 1 = spin up = +1 (src/initialize_detail.h:118-121)
+
+
+def _on_spin(op, s, spins):
+    """op acting on spin s (basis bit s) of `spins` spins: bit s is factor spins-1-s of the Kronecker chain."""
+    out = np.eye(1)
+    for f in range(spins - 1, -1, -1):
+        out = np.kron(out, op if f == s else np.eye(2))
+    return out
+
+
+def _tfim_kron(J, h):
+    spins = J.shape[0]
+    N = 1 << spins
+    H = np.zeros((N, N))
+    for i in range(spins):
+        for j in range(i + 1, spins):
+            H += J[i, j] * _on_spin(_Z_BIT, i, spins) @ _on_spin(_Z_BIT, j, spins)
+        H += h * _on_spin(_X, i, spins)
+    return H
+
+
+@pytest.mark.parametrize("spins,seed,h", [(3, 0, 1.0), (4, 4, 1.0), (6, 1, 0.7)])
+def test_tfim_hamiltonian_equals_kronecker_construction(oracle_mod, spins, seed, h):
+    J = oracle_mod.tfim_couplings(spins, seed)
+    got = oracle_mod.tfim_hamiltonian(spins, J, h).dense()
+    want = _tfim_kron(J, h)
+    assert np.max(np.abs(got.imag)) == 0.0
+    assert np.max(np.abs(got.real - want)) <= 1e-14
+    # and with couplings that are not the seeded ones (a transposed-index or sign slip shows here)
+    rng = np.random.default_rng(spins)
+    J2 = np.triu(rng.normal(size=(spins, spins)), 1)
+    assert np.max(np.abs(oracle_mod.tfim_hamiltonian(spins, J2, h).dense().real - _tfim_kron(J2, h))) <= 1e-14
+
+
+def test_tfim_hamiltonian_closed_form_spectra(oracle_mod):
+    spins = 5
+    # J = 0: h sum_i X_i has eigenvalues h (spins - 2k) with multiplicity C(spins, k)
+    H = oracle_mod.tfim_hamiltonian(spins, np.zeros((spins, spins)), 0.8).dense().real
+    ev = np.sort(np.linalg.eigvalsh(H))
+    want = np.sort(np.concatenate([[0.8 * (spins - 2 * k)] * math.comb(spins, k) for k in range(spins + 1)]))
+    assert np.max(np.abs(ev - want)) <= 1e-12
+    # h = 0: diagonal, entry of basis state b = sum_{i<j} J_ij s_i s_j with s = 2 bit - 1
+    J = oracle_mod.tfim_couplings(spins, 3)
+    H = oracle_mod.tfim_hamiltonian(spins, J, 0.0).dense().real
+    assert np.count_nonzero(H - np.diag(np.diag(H))) == 0
+    for b in (0, 1, 6, 19, 31):
+        s = np.array([2 * ((b >> q) & 1) - 1 for q in range(spins)], float)
+        assert abs(H[b, b] - sum(J[i, j] * s[i] * s[j] for i in range(spins) for j in range(i + 1, spins))) <= 1e-14
+
+
+def test_trace_h_rho_equals_numpy_trace(oracle_mod):
+    rng = np.random.default_rng(7)
+    for spins in (3, 4, 6):
+        N = 1 << spins
+        J = oracle_mod.tfim_couplings(spins, spins)
+        H = oracle_mod.tfim_hamiltonian(spins, J, 1.0)
+        rho = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))  # not Hermitian: Im part counts too
+        p = oracle_mod.spin_glass_problem(spins, spins, L=1)
+        out = np.zeros(2)
+        r = np.ascontiguousarray(rho, np.complex128)
+        s = H._struct()
+        oracle_mod.lib().hx_trace_h_rho(oracle_mod.C.byref(s), oracle_mod._pd(r.view(np.float64)), oracle_mod._pd(out))
+        want = np.trace(_tfim_kron(J, 1.0) @ rho)
+        assert abs(out[0] - want.real) <= 1e-12 * N and abs(out[1] - want.imag) <= 1e-12 * N
+        assert abs(p.energy(rho[None]) - want.real) <= 1e-12 * N
+    # a complex Hermitian H: the imaginary parts of H enter with the right sign
+    N = 8
+    A = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    Hc = A + A.conj().T
+    rho = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    out = np.zeros(2)
+    Hcsr = oracle_mod.Csr.from_dense(Hc)  # kept alive: the struct points into its arrays
+    s = Hcsr._struct()
+    r = np.ascontiguousarray(rho, np.complex128)
+    oracle_mod.lib().hx_trace_h_rho(oracle_mod.C.byref(s), oracle_mod._pd(r.view(np.float64)), oracle_mod._pd(out))
+    want = np.trace(Hc @ rho)
+    assert abs(out[0] - want.real) <= 1e-12 and abs(out[1] - want.imag) <= 1e-12
+
+
+def test_normal_draws_have_standard_normal_moments(oracle_mod):
+    # hx_normal (Box-Muller over SplitMix64): sample moments of 2e5 draws against N(0, 1), and a
+    # Kolmogorov-Smirnov test; tolerances are ~5 standard errors of each estimator
+    from scipy import stats
+    r = oracle_mod.Rng(20240601)
+    n = 200_000
+    x = np.array([r.normal() for _ in range(n)])
+    assert abs(x.mean()) < 5 / math.sqrt(n)
+    assert abs(x.var() - 1.0) < 5 * math.sqrt(2.0 / n)
+    assert abs(stats.kurtosis(x, fisher=False) - 3.0) < 5 * math.sqrt(24.0 / n)
+    assert abs(stats.skew(x)) < 5 * math.sqrt(6.0 / n)
+    assert stats.kstest(x, "norm").pvalue > 1e-4
+    # the couplings use consecutive draws, upper triangle only, row by row
+    J = oracle_mod.tfim_couplings(4, 11)
+    r2 = oracle_mod.Rng(11)
+    want = np.zeros((4, 4))
+    for i in range(4):
+        for j in range(i + 1, 4):
+            want[i, j] = r2.normal()
+    assert np.array_equal(J, want)
