@@ -1434,6 +1434,9 @@ struct helix_hip_plan {
 	double2* kappa = nullptr;
 	int32_t* kslot = nullptr;
 	int kappa_count = 0;
+	// trace staging of helix_hip_step, allocated on the first recording call and reused while it suffices
+	double* trace = nullptr;
+	size_t trace_doubles = 0;
 };
 
 namespace helix_amd_mlp {
@@ -1862,6 +1865,125 @@ int launch_error(const char* what)
 	return 0;
 }
 
+// Trace records of one helix_hip_step call: k_record writes each record into a device staging buffer of `cap`
+// records (cap full blocks [N][N][2], then cap diagonals [N]), which is copied to the caller's arrays whenever it
+// fills and at the end, so a call of any length needs only a bounded buffer.
+struct TraceStaging {
+	uint64_t nrec = 0;      // records of the call
+	uint64_t cap = 0;       // records the staging buffer holds
+	double* dev = nullptr;  // staging buffer
+	bool rho = false;
+	bool diag = false;
+};
+
+constexpr size_t kTraceStagingBytes = size_t(64) << 20;  // plan-owned staging of helix_hip_step
+
+size_t record_doubles(const helix_hip_plan* p, bool rho, bool diag)
+{
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	return (rho ? 2 * n2 : 0) + (diag ? static_cast<size_t>(p->N) : 0);
+}
+
+int trace_staging_on(const helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, const double* rho_trace,
+	const double* diag_trace, double* dev, size_t dev_doubles, TraceStaging& st)
+{
+	st = TraceStaging{};
+	if(record_every == 0 || (!rho_trace && !diag_trace)) return 0;
+	st.nrec = nsteps / record_every;
+	if(st.nrec == 0) return 0;
+	st.rho = rho_trace != nullptr;
+	st.diag = diag_trace != nullptr;
+	const size_t per = record_doubles(p, st.rho, st.diag);
+	st.cap = std::min<uint64_t>(st.nrec, dev_doubles / per);
+	if(st.cap == 0)
+		return fail(HELIX_HIP_EINVAL, "one trace record of N = " + std::to_string(p->N) + " needs " +
+			std::to_string(per * sizeof(double)) + " bytes, more than the " + std::to_string(dev_doubles * sizeof(double)) +
+			"-byte staging buffer");
+	st.dev = dev;
+	return 0;
+}
+
+int plan_trace_staging(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, const double* rho_trace,
+	const double* diag_trace, TraceStaging& st)
+{
+	st = TraceStaging{};
+	if(record_every == 0 || (!rho_trace && !diag_trace) || nsteps / record_every == 0) return 0;
+	const size_t per = record_doubles(p, rho_trace != nullptr, diag_trace != nullptr);
+	const uint64_t want = std::min<uint64_t>(nsteps / record_every, std::max<size_t>(1, kTraceStagingBytes / (per * 8)));
+	if(p->trace_doubles < want * per) {
+		if(p->trace) {
+			HX_HIP(hipStreamSynchronize(p->stream));
+			(void)hipFree(p->trace);
+			p->device_bytes -= p->trace_doubles * sizeof(double);
+			p->trace = nullptr;
+			p->trace_doubles = 0;
+		}
+		void* d = nullptr;
+		if(hipMalloc(&d, want * per * sizeof(double)) != hipSuccess) return fail(HELIX_HIP_ENOMEM, "trace staging buffer");
+		p->trace = static_cast<double*>(d);
+		p->trace_doubles = want * per;
+		p->device_bytes += p->trace_doubles * sizeof(double);
+	}
+	return trace_staging_on(p, nsteps, record_every, rho_trace, diag_trace, p->trace, p->trace_doubles, st);
+}
+
+int flush_records(helix_hip_plan* p, const TraceStaging& st, uint64_t first, uint64_t count, double* rho_trace,
+	double* diag_trace)
+{
+	if(count == 0) return 0;
+	if(hipError_t e = hipStreamSynchronize(p->stream); e != hipSuccess) return hip_fail(e, "step");
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	if(st.rho) HX_HIP(hipMemcpy(rho_trace + first * n2 * 2, st.dev, count * n2 * 16, hipMemcpyDeviceToHost));
+	if(st.diag)
+		HX_HIP(hipMemcpy(diag_trace + first * p->N, st.dev + (st.rho ? st.cap * n2 * 2 : 0), count * p->N * sizeof(double),
+			hipMemcpyDeviceToHost));
+	return 0;
+}
+
+// nsteps steps with the records of `st`; graphs: replay 16-step chunks from the plan's hipGraph where they fit
+// (capturing it on first use), else launch every stage eagerly.  Returns after the device finished.
+int step_impl(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, double* rho_trace, double* diag_trace,
+	const TraceStaging& st, bool graphs)
+{
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	uint64_t r = 0, s = 0, flushed = 0;
+	while(s < nsteps) {
+		// steps up to the next record point (or the end), in graph chunks where they fit
+		const uint64_t next = (r < st.nrec) ? (r + 1) * record_every : nsteps;
+		while(s < next) {
+			if(graphs && p->order >= 2 && next - s >= kChunkSteps) {
+				if(int rc = enqueue_chunk(p)) return rc;
+				s += kChunkSteps;
+			} else {
+				enqueue_step(p);
+				++s;
+			}
+		}
+		if(r < st.nrec && s == (r + 1) * record_every) {
+			const uint64_t slot = r - flushed;
+			double* ro = st.rho ? st.dev + slot * n2 * 2 : nullptr;
+			double* dg = st.diag ? st.dev + (st.rho ? st.cap * n2 * 2 : 0) + slot * p->N : nullptr;
+			const unsigned g = grid_for(n2);
+			const int hm = p->herm ? 1 : 0;
+			if(p->precision == HELIX_HIP_FP64)
+				hipLaunchKernelGGL((k_record<double>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, nullptr, p->logN, hm, ro, dg);
+			else
+				hipLaunchKernelGGL((k_record<float>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, p->rho0_f64, p->logN, hm,
+					ro, dg);
+			++r;
+			if(r - flushed == st.cap) {
+				if(int rc = launch_error("record launch")) return rc;
+				if(int rc = flush_records(p, st, flushed, st.cap, rho_trace, diag_trace)) return rc;
+				flushed = r;
+			}
+		}
+		if(int rc = launch_error("step launch")) return rc;
+	}
+	if(int rc = launch_error("step launch")) return rc;
+	if(hipError_t e = hipStreamSynchronize(p->stream); e != hipSuccess) return hip_fail(e, "step");
+	return flush_records(p, st, flushed, r - flushed, rho_trace, diag_trace);
+}
+
 // One N x N complex block (interleaved doubles) into the Hermitian layout; false when some
 // |rho[j][i] - conj(rho[i][j])| exceeds tol.
 bool pack_block(const double* full, int logN, double* packed, double tol)
@@ -2261,6 +2383,7 @@ void helix_hip_plan_destroy(helix_hip_plan* p)
 	drop_chunk_graph(p);
 	for(hipEvent_t e : p->events) (void)hipEventDestroy(e);
 	for(void* q : p->allocations) (void)hipFree(q);
+	if(p->trace) (void)hipFree(p->trace);
 	if(p->stream) (void)hipStreamDestroy(p->stream);
 	delete p;
 }
@@ -2303,56 +2426,9 @@ int helix_hip_rhs(helix_hip_plan* p, double* out)
 int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, double* rho_trace, double* diag_trace)
 {
 	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
-	const bool record = record_every > 0 && (rho_trace || diag_trace);
-	const uint64_t nrec = record ? nsteps / record_every : 0;
-	const size_t n2 = static_cast<size_t>(p->N) * p->N;
-	double* d_rho = nullptr;
-	double* d_diag = nullptr;
-	auto done = [&](int rc) {
-		if(d_rho) (void)hipFree(d_rho);
-		if(d_diag) (void)hipFree(d_diag);
-		return rc;
-	};
-	if(nrec && rho_trace && hipMalloc(&d_rho, nrec * n2 * 16) != hipSuccess)
-		return done(fail(HELIX_HIP_ENOMEM, "trace buffer"));
-	if(nrec && diag_trace && hipMalloc(&d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double)) != hipSuccess)
-		return done(fail(HELIX_HIP_ENOMEM, "trace buffer"));
-	uint64_t r = 0;
-	uint64_t s = 0;
-	while(s < nsteps) {
-		// steps up to the next record point (or the end), in graph chunks where they fit
-		const uint64_t next = (nrec && r < nrec) ? (r + 1) * record_every : nsteps;
-		while(s < next) {
-			if(p->order >= 2 && next - s >= kChunkSteps) {
-				if(int rc = enqueue_chunk(p)) return done(rc);
-				s += kChunkSteps;
-			} else {
-				enqueue_step(p);
-				++s;
-			}
-		}
-		if(nrec && r < nrec && s == (r + 1) * record_every) {
-			const unsigned g = grid_for(n2);
-			double* ro = d_rho ? d_rho + r * n2 * 2 : nullptr;
-			double* dg = d_diag ? d_diag + r * static_cast<size_t>(p->N) : nullptr;
-			const int hm = p->herm ? 1 : 0;
-			if(p->precision == HELIX_HIP_FP64)
-				hipLaunchKernelGGL((k_record<double>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, nullptr, p->logN, hm, ro, dg);
-			else
-				hipLaunchKernelGGL((k_record<float>), dim3(g), dim3(kBlock), 0, p->stream, p->rho, p->rho0_f64, p->logN, hm,
-					ro, dg);
-			++r;
-		}
-		if(int rc = launch_error("step launch")) return done(rc);
-	}
-	if(int rc = launch_error("step launch")) return done(rc);
-	if(hipError_t e = hipStreamSynchronize(p->stream); e != hipSuccess) return done(hip_fail(e, "step"));
-	if(d_rho && hipMemcpy(rho_trace, d_rho, nrec * n2 * 16, hipMemcpyDeviceToHost) != hipSuccess)
-		return done(fail(HELIX_HIP_ESTATE, "trace copy failed"));
-	if(d_diag && hipMemcpy(diag_trace, d_diag, nrec * static_cast<size_t>(p->N) * sizeof(double), hipMemcpyDeviceToHost) !=
-			hipSuccess)
-		return done(fail(HELIX_HIP_ESTATE, "trace copy failed"));
-	return done(0);
+	TraceStaging st;
+	if(int rc = plan_trace_staging(p, nsteps, record_every, rho_trace, diag_trace, st)) return rc;
+	return step_impl(p, nsteps, record_every, rho_trace, diag_trace, st, true);
 }
 
 int helix_hip_get_tier0(helix_hip_plan* p, double* out)
