@@ -28,7 +28,7 @@ HIP_SYMBOLS = [
     "helix_hip_last_error", "helix_hip_batch_create", "helix_hip_batch_set_initial", "helix_hip_batch_step",
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
-    "helix_hip_plan_layout", "helix_hip_batch_layout",
+    "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
@@ -97,6 +97,7 @@ def hip():
         L.helix_hip_get_state.argtypes = [vp, _dp]
         L.helix_hip_rhs.argtypes = [vp, _dp]
         L.helix_hip_step.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
+        L.helix_hip_step_noalloc.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
         L.helix_hip_get_tier0.argtypes = [vp, _dp]
         L.helix_hip_time_steps.argtypes = [vp, C.c_uint64, _dp, _dp]
         L.helix_hip_plan_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),

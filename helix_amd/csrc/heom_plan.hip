@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1402,6 +1403,14 @@ __global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const doub
 
 using namespace helix_amd;
 
+// Device scratch of helix_hip_step_noalloc (include/helix_hip.h): the staging buffer of its trace records.  Each GPU
+// the library runs on holds its own copy of this array until the process ends; host code never passes the
+// array's name to a kernel or a copy (on the host it only stands in for the symbol), it uses the device address
+// that resolve_scratch() obtained with hipGetSymbolAddress on that device.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 struct helix_hip_plan {
 	int device = 0;
 	hipStream_t stream = nullptr;
@@ -1437,6 +1446,8 @@ struct helix_hip_plan {
 	// trace staging of helix_hip_step, allocated on the first recording call and reused while it suffices
 	double* trace = nullptr;
 	size_t trace_doubles = 0;
+	// this device's copy of `scratch` (helix_hip_step_noalloc)
+	double* scratch = nullptr;
 };
 
 namespace helix_amd_mlp {
@@ -1865,6 +1876,29 @@ int launch_error(const char* what)
 	return 0;
 }
 
+// Device address of `scratch` on `device`, resolved once per device with that device current (each GPU holds its
+// own copy of the array at its own address).
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int resolve_scratch(int device, double** out)
+{
+	*out = nullptr;
+	if(device < 0 || device >= kMaxScratchDevices) return fail(HELIX_HIP_EINVAL, "device index out of range");
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if(!g_scratch_ptr[device]) {
+		int cur = -1;
+		HX_HIP(hipGetDevice(&cur));
+		if(cur != device) return fail(HELIX_HIP_ESTATE, "resolve_scratch: the plan's device is not current");
+		void* ptr = nullptr;
+		HX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		g_scratch_ptr[device] = static_cast<double*>(ptr);
+	}
+	*out = g_scratch_ptr[device];
+	return 0;
+}
+
 // Trace records of one helix_hip_step call: k_record writes each record into a device staging buffer of `cap`
 // records (cap full blocks [N][N][2], then cap diagonals [N]), which is copied to the caller's arrays whenever it
 // fills and at the end, so a call of any length needs only a bounded buffer.
@@ -2218,6 +2252,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 		return rc;
 	};
 	p->device = device;
+	if(int rc = resolve_scratch(device, &p->scratch)) return bail(rc);
 	p->N = pr->n;
 	p->logN = ilog2_exact(pr->n);
 	p->H = pr->n_ado;
@@ -2429,6 +2464,17 @@ int helix_hip_step(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, do
 	TraceStaging st;
 	if(int rc = plan_trace_staging(p, nsteps, record_every, rho_trace, diag_trace, st)) return rc;
 	return step_impl(p, nsteps, record_every, rho_trace, diag_trace, st, true);
+}
+
+int helix_hip_step_noalloc(helix_hip_plan* p, uint64_t nsteps, uint64_t record_every, double* rho_trace,
+	double* diag_trace)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	if(!p->scratch) return fail(HELIX_HIP_ESTATE, "the device scratch of this plan's device was not resolved");
+	TraceStaging st;
+	if(int rc = trace_staging_on(p, nsteps, record_every, rho_trace, diag_trace, p->scratch, kScratchDoubles, st))
+		return rc;
+	return step_impl(p, nsteps, record_every, rho_trace, diag_trace, st, false);
 }
 
 int helix_hip_get_tier0(helix_hip_plan* p, double* out)

@@ -138,6 +138,17 @@ int helix_hip_rhs(helix_hip_plan* plan, double* out);
 int helix_hip_step(helix_hip_plan* plan, uint64_t nsteps, uint64_t record_every, double* rho_trace,
 	double* diag_trace);
 
+/* helix_hip_step without any device allocation.  No counterpart in the reference: it serves callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, steps and records as helix_hip_step, bit for bit; every stage is launched eagerly (no hipGraph
+ * is captured), and the trace records are staged in a 32 MiB device array of the library (one copy per GPU,
+ * held until the process ends) and copied out whenever it fills, so nsteps is unbounded but one record
+ * (2 N^2 doubles with rho_trace, plus N with diag_trace) must fit in 32 MiB: N <= 1024 with rho_trace, else
+ * HELIX_HIP_EINVAL.  Every call on one GPU uses that GPU's array: calls that overlap on one GPU (several host
+ * threads) race. */
+int helix_hip_step_noalloc(helix_hip_plan* plan, uint64_t nsteps, uint64_t record_every, double* rho_trace,
+	double* diag_trace);
+
 /* Tier-0 block (the reduced density rho_S) as [N][N][2] row-major complex128
  * (ResultExtractor::final_reduced_density, src/library/result_extractor.cu:35-89). */
 int helix_hip_get_tier0(helix_hip_plan* plan, double* out);
