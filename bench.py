@@ -3,15 +3,16 @@
 
 Workload (default): the plain FP64 transverse-field spin glass with 6 spins (N = 64), one sigma^z
 Drude-Lorentz bath per spin (lambda 0.1, gamma 1, T 1, Pade K = 1: M = 12 modes), tiers 0..5
-(C(17,5) = 6188 ADOs of 64x64 complex128, 405.5 MB per state vector), Taylor-4 steps of dt = 0.005,
-synthetic seeded couplings (seed 1).  One "step" advances every ADO by one full 4-stage step.
+(C(17,5) = 6188 ADOs of 64x64 complex128), Taylor-4 steps of dt = 0.005, synthetic seeded couplings
+(seed 1).  One "step" advances every ADO by one full 4-stage step.
 
 value = ADO-steps/s = n_ado * steps / wall time of the timed steps (state resident in HBM).
-roofline: the fused stage kernel, algorithmic bytes 14 N^2 s per ADO-step (s = bytes per complex),
-divided over the m stage launches of a step, against the launch durations measured with HIP events on
-the plan's stream.  The device stores Hermitian ADOs in N(N+1)/2 elements, so the bytes it must move are
-(N+1)/(2N) of that figure (reported as roofline.stored_fraction).  cpu_baseline: the FP64 C oracle
-(OpenMP) timed on this host for one step of the same workload.
+roofline: the fused stage kernel.  The device stores each Hermitian ADO as P = N(N+1)/2 elements, so the
+algorithmic bytes per ADO-step are 14 P s (s = bytes per complex; 14 blocks per step as counted for the
+reference's scheme), divided over the m stage launches of a step, against the launch durations measured
+with HIP events on the plan's stream (roofline.basis).  The full-block figure 14 N^2 s is reported beside
+it as an effective rate only.  cpu_baseline: the FP64 C oracle (OpenMP) timed on this host for one step
+of the same workload.
 """
 from __future__ import annotations
 
@@ -36,11 +37,26 @@ WORKLOADS = {
 
 
 def _cpu_cores():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the CPU share a GPU gets on its host), else every
+    core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
     try:
-        n = len(os.sched_getaffinity(0))
+        return max(1, len(os.sched_getaffinity(0)))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(spec, steps: int = 1):
@@ -54,6 +70,7 @@ def cpu_baseline(spec, steps: int = 1):
     p.taylor(r, dt, steps)
     sec = time.perf_counter() - t0
     return {"value": p.H * steps / sec, "unit": "ADO-steps/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{steps} Taylor-4 step(s) of the full workload ({p.H} ADOs, N={p.N}) in {sec:.2f} s"}
 
 
@@ -72,6 +89,7 @@ def _profiled_traffic(workload, precision, layout):
 
 
 def _time_single(model_kwargs, steps, warmup=10):
+    """ADO-steps/s of `steps` timed steps after `warmup`, and rho_S after all warmup + steps steps."""
     import helix_amd as hx
     m = hx.Model.spin_glass(**model_kwargs)
     plan = hx.Plan(m)
@@ -80,11 +98,13 @@ def _time_single(model_kwargs, steps, warmup=10):
     t0 = time.perf_counter()
     plan.step(steps)
     sec = time.perf_counter() - t0
+    rho_s = plan.tier0()
     plan.close()
-    return m.H * steps / sec
+    return m.H * steps / sec, rho_s
 
 
 def _time_batch(precision, count, steps, warmup=2):
+    """ADO-steps/s of the batched workload, its info, and every instance's rho_S after warmup + steps steps."""
     import helix_amd as hx
     models = [hx.Model.spin_glass_random(1000 + i, dt=0.01, precision=precision) for i in range(count)]
     b = hx.Batch(models, graph=True, autotune=True)
@@ -94,24 +114,35 @@ def _time_batch(precision, count, steps, warmup=2):
     b.step(steps)
     sec = time.perf_counter() - t0
     info = b.info()
+    rho_s = [b.tier0(i) for i in range(count)]
     b.close()
-    return info["total_ado"] * steps / sec, info
+    return info["total_ado"] * steps / sec, info, rho_s
+
+
+def _max_abs(a, b):
+    import numpy as np
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
 
 
 def measure_forms(args, fp64_value):
     """The chosen forms beside the plain FP64 line: complex64 and split-FP32 on the same workload, and the
-    512-instance batched workload in FP64 and complex64 (ADO-steps/s and the ratio to plain FP64 of the
-    same workload)."""
+    512-instance batched workload in FP64 and complex64.  Each form reports its ADO-steps/s, the ratio to plain
+    FP64 on the same workload, and max|delta rho_S| against the FP64 run of the same workload from the same
+    initial state after the same number of steps (warmup + timed)."""
     spins, seed, L, dt, lam, gamma, T, K = WORKLOADS[args.workload]
     kw = dict(spins=spins, seed=seed, lam=lam, gamma=gamma, T=T, K=K, L=L, dt=dt)
     forms = {}
+    _, ref = _time_single(dict(kw, precision="fp64"), args.form_steps)
     for prec in ("fp32", "split_fp32"):
-        v = _time_single(dict(kw, precision=prec), args.form_steps)
-        forms[prec] = {"value": v, "x_plain_fp64": v / fp64_value}
-    v64, info = _time_batch("fp64", 512, args.batch_steps)
-    v32, _ = _time_batch("fp32", 512, args.batch_steps)
+        v, rho_s = _time_single(dict(kw, precision=prec), args.form_steps)
+        forms[prec] = {"value": v, "x_plain_fp64": v / fp64_value, "max_abs_drho_S": _max_abs(rho_s, ref),
+                       "steps_compared": 10 + args.form_steps}
+    v64, info, r64 = _time_batch("fp64", 512, args.batch_steps)
+    v32, _, r32 = _time_batch("fp32", 512, args.batch_steps)
     forms["batched_512"] = {"n_ado": info["total_ado"], "fp64": v64, "complex64": v32,
-                            "complex64_x_batched_fp64": v32 / v64, "complex64_x_plain_fp64_n64": v32 / fp64_value}
+                            "complex64_x_batched_fp64": v32 / v64, "complex64_x_plain_fp64_n64": v32 / fp64_value,
+                            "complex64_max_abs_drho_S": max(_max_abs(a, b) for a, b in zip(r32, r64)),
+                            "steps_compared": 2 + args.batch_steps}
     print(f"# forms: {json.dumps(forms)}", file=sys.stderr, flush=True)
     return forms
 
@@ -235,7 +266,8 @@ def bench_batched(args):
     batch.step(args.steps)
     sec = time.perf_counter() - t0
     s_bytes = 16 if args.precision == "fp64" else 8
-    alg_bytes_step = sum(14 * m.N * m.N * s_bytes * m.H for m in models)
+    herm = batch.layout() == "hermitian"
+    alg_bytes_step = sum(14 * ((m.N * (m.N + 1)) // 2 if herm else m.N * m.N) * s_bytes * m.H for m in models)
     ms_step = sec * 1e3 / args.steps
     achieved = alg_bytes_step / (ms_step * 1e-3) / 1e9
     line = {
@@ -246,7 +278,9 @@ def bench_batched(args):
         "config": {"workload": f"batched_{len(models)}", "instances": len(models), "n_ado": info["total_ado"],
                    "precision": args.precision, "threads": info["threads"], "graph": info["graph"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_stage"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_stage",
+                     "basis": "14 P s bytes per ADO-step, P stored elements per ADO of the batch's "
+                              + ("Hermitian" if herm else "full") + " layout"},
     }
     print(json.dumps(line), flush=True)
     batch.close()
@@ -306,12 +340,14 @@ def main(argv=None):
     stage_ms, total_ms = plan.time_steps(max(1, args.timed_steps))
     s_bytes = 16 if args.precision == "fp64" else 8
     n2 = model.N * model.N
-    alg_bytes_step = 14 * n2 * s_bytes * model.H
-    per_launch = alg_bytes_step / model.order
-    mean_launch_ms = float(sum(stage_ms) / len(stage_ms))
-    achieved = per_launch / (mean_launch_ms * 1e-3) / 1e9
     info = plan.info()
     layout = plan.layout()
+    stored = (model.N * (model.N + 1)) // 2 if layout == "hermitian" else n2
+    alg_bytes_step = 14 * stored * s_bytes * model.H
+    per_launch = alg_bytes_step / model.order
+    full_per_launch = 14 * n2 * s_bytes * model.H / model.order
+    mean_launch_ms = float(sum(stage_ms) / len(stage_ms))
+    achieved = per_launch / (mean_launch_ms * 1e-3) / 1e9
     line = {
         "metric": "ADO-steps/sec on 1 MI355X",
         "value": value,
@@ -331,17 +367,21 @@ def main(argv=None):
                    "layout": layout},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "basis": f"14 P s bytes per ADO-step over the {layout} layout's P = "
+                              f"{stored} stored elements per ADO, s = {s_bytes}",
                      "kernel": ("k_stage_dense" if args.dense_hamiltonian else "k_stage_tile")
                      + " (fused Liouville RHS + Taylor epilogue)",
                      "alg_bytes_per_launch": per_launch, "mean_launch_ms": mean_launch_ms,
                      "stage_ms": [float(x) for x in stage_ms],
-                     "stored_fraction": (model.N + 1) / (2 * model.N) if layout == "hermitian" else 1.0},
+                     "effective_full_block_GBs": full_per_launch / (mean_launch_ms * 1e-3) / 1e9},
         "launches_per_step": info["launches_per_step"],
     }
     traffic = None if args.dense_hamiltonian else _profiled_traffic(args.workload, args.precision, layout)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic
-        line["roofline"]["traffic_source"] = "profiles/stage_traffic_spin_glass_n64_L5.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+        line["roofline"]["traffic_source"] = ("committed constant, not measured in this run: mean HBM bytes per stage launch "
+                                              "from profiles/stage_traffic_spin_glass_n64_L5.json (rocprofv3 "
+                                              "FETCH_SIZE/WRITE_SIZE of this kernel, scaled by a device copy)")
     if not args.no_forms and args.workload == "spin_glass_n64_L5" and args.precision == "fp64" \
             and not args.dense_hamiltonian:
         line["forms"] = measure_forms(args, value)

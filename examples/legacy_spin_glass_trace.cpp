@@ -1,5 +1,10 @@
-// Documented drop-in usage (same program as the reference's examples/cpp/legacy_spin_glass.cpp):
-// the legacy spin glass through helix::HEOMSolver().run(system, hierarchy, options).
+// The reference's documented usage program (examples/cpp/legacy_spin_glass.cpp), edited: the step count is
+// taken from argv[1] and the trace of rho_S is printed.  The legacy spin glass through
+// helix::HEOMSolver().run(system, hierarchy, options).  The unedited program is built from the reference's
+// own file by helix_amd/build.py when that file is present.
+//
+// Derived from HELIX, MIT License: Copyright (c) 2015 Masashi Tsuchimoto and Yoshitaka Tanimura;
+// Copyright (c) 2026 Ye Jun; Copyright (c) 2026 HELIX contributors.
 #include <helix/helix.h>
 
 #include <cstdlib>

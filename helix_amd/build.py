@@ -3,7 +3,8 @@
   libhelix_hip.so  hand-written gfx950 HIP kernels + the C-ABI of include/helix_hip.h  (hipcc)
   libhelix.so      C++17 host: set-up (include/helix_setup.h) and the public API of <helix/helix.h>,
                    calling the kernels only through the C-ABI                         (g++)
-  helix_example    the documented drop-in usage, examples/legacy_spin_glass.cpp        (g++)
+  helix_example    the documented drop-in usage with a trace print, examples/legacy_spin_glass_trace.cpp (g++)
+  helix_reference_example  the reference's examples/cpp/legacy_spin_glass.cpp compiled unchanged, when present
 
 Run ``python -m helix_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -19,6 +20,7 @@ ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
 CSRC = os.path.join(PKG, "csrc")
 ARCH = "gfx950"
+REFERENCE_EXAMPLE = "/root/reference/examples/cpp/legacy_spin_glass.cpp"
 
 HIP_SOURCES = ["heom_plan.hip", "mlp.hip"]
 HOST_SOURCES = ["host/heom_setup.cpp", "host/setup_capi.cpp", "host/helix_api.cpp", "host/helix_io.cpp"]
@@ -72,7 +74,7 @@ def build(verbose: bool = False) -> dict:
           f"-L{PKG}", "-lhelix", "-lhelix_hip", "-Wl,-rpath,$ORIGIN"])
     out["helix_python_module"] = pymod
 
-    for src, name in (("legacy_spin_glass.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint"),
+    for src, name in (("legacy_spin_glass_trace.cpp", "helix_example"), ("trace_and_checkpoint.cpp", "helix_trace_checkpoint"),
                       ("sparse_system.cpp", "helix_sparse_system")):
         example = os.path.join(ROOT, "examples", src)
         if os.path.exists(example):
@@ -80,6 +82,13 @@ def build(verbose: bool = False) -> dict:
             _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", example, "-o", exe, f"-L{PKG}", "-lhelix", "-lhelix_hip",
                   "-Wl,-rpath,$ORIGIN"])
             out[name] = exe
+    # the reference's own example program, compiled unchanged from where it lies (a drop-in check of the public
+    # headers; nothing is copied): only in a tree next to the reference, its binary then travels with the tree
+    if os.path.exists(REFERENCE_EXAMPLE):
+        exe = os.path.join(PKG, "helix_reference_example")
+        _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", REFERENCE_EXAMPLE, "-o", exe, f"-L{PKG}", "-lhelix",
+              "-lhelix_hip", "-Wl,-rpath,$ORIGIN"])
+        out["helix_reference_example"] = exe
     if verbose:
         for k, v in out.items():
             print(f"built {k}: {v}")

@@ -41,7 +41,7 @@ SETUP_SYMBOLS = [
 
 IO_SYMBOLS = [
     "helix_io_write_energy_trace", "helix_io_hierarchy_hash", "helix_io_checkpoint_write", "helix_io_checkpoint_read",
-    "helix_io_last_error",
+    "helix_io_problem_hash", "helix_io_last_error",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -56,7 +56,7 @@ class CheckpointInfo(C.Structure):
     """Mirror of helix_io_checkpoint."""
     _fields_ = [("n", C.c_int32), ("n_ado", C.c_int32), ("n_modes", C.c_int32), ("n_baths", C.c_int32),
                 ("order", C.c_int32), ("precision", C.c_int32), ("steps", C.c_uint64), ("time", C.c_double),
-                ("dt", C.c_double), ("hierarchy_hash", C.c_uint64)]
+                ("dt", C.c_double), ("hierarchy_hash", C.c_uint64), ("problem_hash", C.c_uint64)]
 
 
 class Problem(C.Structure):
@@ -156,6 +156,8 @@ def host():
         L.helix_io_hierarchy_hash.restype = C.c_uint64
         L.helix_io_checkpoint_write.argtypes = [C.c_char_p, C.POINTER(CheckpointInfo), _dp]
         L.helix_io_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(CheckpointInfo), _dp, C.c_uint64]
+        L.helix_io_problem_hash.argtypes = [C.c_void_p]
+        L.helix_io_problem_hash.restype = C.c_uint64
         L.helix_io_last_error.restype = C.c_char_p
         _host = L
     return _host
@@ -258,6 +260,10 @@ class Model:
     def problem_ptr(self):
         return host().helix_model_problem(self._h)
 
+    def problem_hash(self) -> int:
+        """helix_io_problem_hash of this model's problem descriptor."""
+        return int(host().helix_io_problem_hash(C.cast(self.problem_ptr(), C.c_void_p)))
+
     def rows(self) -> np.ndarray:
         r = np.zeros((self.H, self.M), np.int32)
         host().helix_model_rows(self._h, r.ctypes.data_as(_ip))
@@ -336,7 +342,7 @@ class Plan:
         desc = m.problem_ptr().contents
         info = CheckpointInfo(n=m.N, n_ado=m.H, n_modes=m.M, n_baths=desc.n_baths, order=desc.order,
                               precision=desc.precision, steps=steps, time=time, dt=desc.dt,
-                              hierarchy_hash=hierarchy_hash(m.rows()))
+                              hierarchy_hash=hierarchy_hash(m.rows()), problem_hash=m.problem_hash())
         checkpoint_write(path, info, self.get_state())
 
     def load_checkpoint(self, path: str):
@@ -345,6 +351,11 @@ class Plan:
         info, _ = checkpoint_read(path, with_state=False)
         if (info.n, info.n_ado, info.n_modes, info.hierarchy_hash) != (m.N, m.H, m.M, hierarchy_hash(m.rows())):
             raise HelixError("checkpoint belongs to another problem (Hilbert size or hierarchy differs)")
+        desc = m.problem_ptr().contents
+        if info.order != desc.order or info.dt != desc.dt:
+            raise HelixError("checkpoint was written with another integration order or time step")
+        if info.problem_hash != m.problem_hash():
+            raise HelixError("checkpoint belongs to another problem (bath, Hamiltonian or coupling coefficients differ)")
         _, state = checkpoint_read(path)
         self.set_state(state)
         return info.steps, info.time

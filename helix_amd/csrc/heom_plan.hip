@@ -1798,19 +1798,26 @@ void drop_chunk_graph(helix_hip_plan* p)
 	p->chunk_graph = nullptr;
 }
 
+// Captures and instantiates the chunk graph unless it exists (at plan creation, and again after a change of the
+// stage arguments), so that no step pays for it.
+int capture_chunk(helix_hip_plan* p)
+{
+	if(p->chunk_exec) return 0;
+	HX_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+	for(uint64_t s = 0; s < kChunkSteps; ++s) enqueue_step(p);
+	hipError_t e = hipStreamEndCapture(p->stream, &p->chunk_graph);
+	if(e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+	e = hipGraphInstantiate(&p->chunk_exec, p->chunk_graph, nullptr, nullptr, 0);
+	if(e != hipSuccess) {
+		drop_chunk_graph(p);
+		return hip_fail(e, "hipGraphInstantiate");
+	}
+	return 0;
+}
+
 int enqueue_chunk(helix_hip_plan* p)
 {
-	if(!p->chunk_exec) {
-		HX_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
-		for(uint64_t s = 0; s < kChunkSteps; ++s) enqueue_step(p);
-		hipError_t e = hipStreamEndCapture(p->stream, &p->chunk_graph);
-		if(e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-		e = hipGraphInstantiate(&p->chunk_exec, p->chunk_graph, nullptr, nullptr, 0);
-		if(e != hipSuccess) {
-			drop_chunk_graph(p);
-			return hip_fail(e, "hipGraphInstantiate");
-		}
-	}
+	if(int rc = capture_chunk(p)) return rc;
 	HX_HIP(hipGraphLaunch(p->chunk_exec, p->stream));
 	return 0;
 }
@@ -2407,6 +2414,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	if(p->rho0_f64) HX_HIP(hipMemsetAsync(p->rho0_f64, 0, p->pe * 16, p->stream));
 	HX_HIP(hipStreamSynchronize(p->stream));
 	if((rc = autotune_threads(p))) return bail(rc);
+	if(p->order >= 2 && (rc = capture_chunk(p))) return bail(rc);
 	*out = p;
 	return 0;
 }
@@ -2537,7 +2545,7 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 	if(!mlp) {
 		p->base.kappa = nullptr;
 		p->base.kslot = nullptr;
-		return 0;
+		return p->order >= 2 ? capture_chunk(p) : 0;
 	}
 	if(!ados || count <= 0 || extra_dim < 0 || (extra_dim > 0 && !extra)) return fail(HELIX_HIP_EINVAL, "bad closure");
 	if(p->herm) return fail(HELIX_HIP_EUNSUPPORTED, "the learned closure needs a plan in HELIX_HIP_LAYOUT_FULL");
@@ -2576,7 +2584,7 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 	if(dextra) (void)hipFree(dextra);
 	p->base.kappa = p->kappa;
 	p->base.kslot = p->kslot;
-	return 0;
+	return p->order >= 2 ? capture_chunk(p) : 0;
 }
 
 int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int count, const double* kappa)
@@ -2587,7 +2595,7 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 	if(!kappa) {
 		p->base.kappa = nullptr;
 		p->base.kslot = nullptr;
-		return 0;
+		return p->order >= 2 ? capture_chunk(p) : 0;
 	}
 	if(!ados || count <= 0) return fail(HELIX_HIP_EINVAL, "bad closure rates");
 	if(p->herm) return fail(HELIX_HIP_EUNSUPPORTED, "closure rates need a plan in HELIX_HIP_LAYOUT_FULL");
@@ -2610,7 +2618,7 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 	HX_HIP(hipMemcpy(p->kappa, kappa, sizeof(double2) * rows, hipMemcpyHostToDevice));
 	p->base.kappa = p->kappa;
 	p->base.kslot = p->kslot;
-	return 0;
+	return p->order >= 2 ? capture_chunk(p) : 0;
 }
 
 int helix_hip_plan_info(const helix_hip_plan* p, size_t* device_bytes, int* launches_per_step, int* uses_flip_coupling,

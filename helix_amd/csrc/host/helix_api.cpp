@@ -486,6 +486,12 @@ public:
 			info.hierarchy_hash != mine.hierarchy_hash)
 			throw std::invalid_argument("load_checkpoint: the checkpoint belongs to another problem (Hilbert size or "
 										"hierarchy differs)");
+		if(info.order != mine.order || info.dt != mine.dt)
+			throw std::invalid_argument("load_checkpoint: the checkpoint was written with another integration order or "
+										"time step");
+		if(info.problem_hash != mine.problem_hash)
+			throw std::invalid_argument("load_checkpoint: the checkpoint belongs to another problem (bath, Hamiltonian "
+										"or coupling coefficients differ)");
 		std::vector<double> state(static_cast<std::size_t>(info.n_ado) * info.n * info.n * 2);
 		if(helix_io_checkpoint_read(path.c_str(), &info, state.data(), state.size()) != 0)
 			throw std::runtime_error(std::string("load_checkpoint: ") + helix_io_last_error());
@@ -573,6 +579,7 @@ private:
 		info.dt = options_.timeStep;
 		const auto& rows = problem_->hierarchy().rows;
 		info.hierarchy_hash = helix_io_hierarchy_hash(rows.data(), problem_->ado_count(), problem_->modes());
+		info.problem_hash = helix_io_problem_hash(&problem_->descriptor());
 		return info;
 	}
 

@@ -89,6 +89,50 @@ uint64_t helix_io_hierarchy_hash(const int32_t* rows, int64_t n_ado, int32_t n_m
 	return fnv1a(rows, static_cast<size_t>(n_ado) * static_cast<size_t>(n_modes) * sizeof(int32_t));
 }
 
+uint64_t helix_io_problem_hash(const helix_hip_problem* pr)
+{
+	if(!pr || pr->n <= 0 || pr->n_ado <= 0 || pr->n_modes <= 0 || pr->n_baths <= 0) return 0;
+	const size_t n = static_cast<size_t>(pr->n), H = static_cast<size_t>(pr->n_ado);
+	const size_t M = static_cast<size_t>(pr->n_modes), B = static_cast<size_t>(pr->n_baths);
+	uint64_t h = 1469598103934665603ull;
+	auto add = [&h](const void* p, size_t bytes) {
+		if(p && bytes) h = fnv1a(p, bytes, h);
+	};
+	const int32_t dims[5] = {pr->n, pr->n_ado, pr->n_modes, pr->n_baths, pr->order};
+	add(dims, sizeof(dims));
+	add(&pr->dt, sizeof(double));
+	add(pr->mode_bath, M * sizeof(int32_t));
+	add(pr->down_comm, H * M * 2 * sizeof(double));
+	add(pr->down_anti, H * M * 2 * sizeof(double));
+	add(pr->self_coef, H * 2 * sizeof(double));
+	add(pr->xi, B * 2 * sizeof(double));
+	add(pr->xi_counter, B * 2 * sizeof(double));
+	auto add_spin = [&](const helix_hip_spin_operator& o) {
+		add(o.diag, n * sizeof(double));
+		add(&o.nflips, sizeof(int32_t));
+		if(o.nflips > 0) {
+			add(o.flip_masks, static_cast<size_t>(o.nflips) * sizeof(int32_t));
+			add(o.flip_coefs, static_cast<size_t>(o.nflips) * sizeof(double));
+		}
+	};
+	auto add_csr = [&](const helix_hip_csr_operator& o) {
+		add(&o.nnz, sizeof(int32_t));
+		add(o.row_offsets, (n + 1) * sizeof(int32_t));
+		if(o.nnz > 0) {
+			add(o.columns, static_cast<size_t>(o.nnz) * sizeof(int32_t));
+			add(o.values, static_cast<size_t>(o.nnz) * 2 * sizeof(double));
+		}
+	};
+	add_spin(pr->hamiltonian);
+	add(pr->hamiltonian_dense, n * n * 2 * sizeof(double));
+	if(pr->hamiltonian_csr) add_csr(*pr->hamiltonian_csr);
+	for(size_t b = 0; b < B; ++b) {
+		if(pr->couplings) add_spin(pr->couplings[b]);
+		if(pr->couplings_csr) add_csr(pr->couplings_csr[b]);
+	}
+	return h;
+}
+
 int helix_io_checkpoint_write(const char* path, const helix_io_checkpoint* info, const double* state)
 {
 	if(!path || !info || !state) return io_fail("bad arguments");
@@ -107,7 +151,7 @@ int helix_io_checkpoint_write(const char* path, const helix_io_checkpoint* info,
 	put<uint64_t>(h, 64, info->hierarchy_hash);
 	put<uint64_t>(h, 72, fnv1a(state, payload));
 	put<uint64_t>(h, 80, payload);
-	put<uint64_t>(h, 88, 0);
+	put<uint64_t>(h, 88, info->problem_hash);
 	const std::string tmp = std::string(path) + ".tmp";
 	{
 		std::ofstream os(tmp, std::ios::binary);
@@ -145,6 +189,7 @@ int helix_io_checkpoint_read(const char* path, helix_io_checkpoint* info, double
 	info->time = get<double>(h, 48);
 	info->dt = get<double>(h, 56);
 	info->hierarchy_hash = get<uint64_t>(h, 64);
+	info->problem_hash = get<uint64_t>(h, 88);
 	const uint64_t hash = get<uint64_t>(h, 72);
 	const uint64_t payload = get<uint64_t>(h, 80);
 	if(info->n <= 0 || info->n_ado <= 0 ||

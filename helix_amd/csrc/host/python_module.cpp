@@ -4,6 +4,9 @@
 // (bindings/python/helix_bindings.cpp:25-199), so Python code written against it runs unchanged; the
 // additions of this build's API (Backend::Hip, Precision::SplitSingle, trajectory energies,
 // checkpoints, Bath::drude_lorentz, the spin-glass example) are exposed beside them.
+//
+// The declaration list follows the reference's binding, MIT License: Copyright (c) 2015 Masashi Tsuchimoto and
+// Yoshitaka Tanimura; Copyright (c) 2026 Ye Jun; Copyright (c) 2026 HELIX contributors.
 #include <helix/helix.h>
 
 #include <pybind11/complex.h>

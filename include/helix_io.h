@@ -20,7 +20,7 @@
  *          64  uint64   hierarchy hash (helix_io_hierarchy_hash of the [n_ado][n_modes] rows)
  *          72  uint64   payload hash (FNV-1a 64 of the payload bytes)
  *          80  uint64   payload bytes (= n_ado * n * n * 16)
- *          88  uint64   reserved (0)
+ *          88  uint64   problem hash (helix_io_problem_hash of the run's coefficients and operators; 0 = none)
  *          96  payload  [n_ado][n][n] complex128, row-major, interleaved (re, im)
  * Return codes follow include/helix_hip.h (0 ok, < 0 contract error).
  */
@@ -28,6 +28,8 @@
 #define HELIX_IO_H
 
 #include <stdint.h>
+
+#include "helix_hip.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -46,7 +48,14 @@ typedef struct helix_io_checkpoint {
 	double time;
 	double dt;
 	uint64_t hierarchy_hash;
+	uint64_t problem_hash;
 } helix_io_checkpoint;
+
+/* FNV-1a 64 over everything that defines the propagation of a problem besides its hierarchy rows: the sizes,
+ * Taylor order and dt, the coefficient tables (down_comm, down_anti, self_coef, xi, xi_counter), the
+ * Hamiltonian (diagonal, flips, dense or CSR form) and every coupling (diagonal, flips, CSR form).  A
+ * checkpoint written by one problem is refused by a run whose hash differs (other bath, H, couplings or dt). */
+uint64_t helix_io_problem_hash(const helix_hip_problem* problem);
 
 /* Writes info and the full FP64 state [n_ado][n][n][2]. */
 int helix_io_checkpoint_write(const char* path, const helix_io_checkpoint* info, const double* state);

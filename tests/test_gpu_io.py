@@ -67,3 +67,9 @@ def test_checkpoint_of_another_problem_is_refused(hx, tmp_path):
     a.save_checkpoint(path, steps=0, time=0.0)
     with pytest.raises(hx.HelixError, match="another problem"):
         hx.Plan(hx.Model.spin_glass(3, 1, L=2)).load_checkpoint(path)
+    # same hierarchy, another bath or time step
+    with pytest.raises(hx.HelixError, match="bath, Hamiltonian"):
+        hx.Plan(hx.Model.spin_glass(3, 1, L=3, lam=0.2)).load_checkpoint(path)
+    with pytest.raises(hx.HelixError, match="time step"):
+        hx.Plan(hx.Model.spin_glass(3, 1, L=3, dt=0.02)).load_checkpoint(path)
+    assert hx.Plan(hx.Model.spin_glass(3, 1, L=3)).load_checkpoint(path) == (0, 0.0)

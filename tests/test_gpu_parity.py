@@ -228,6 +228,17 @@ def test_cpp_example_runs(hx):
     assert abs(tr - 1.0) <= 1e-5
 
 
+def test_reference_example_compiled_unchanged_runs(hx):
+    """The reference's examples/cpp/legacy_spin_glass.cpp, compiled unchanged against include/helix by the build
+    (which needs the reference's file next to the tree), runs on the GPU and prints the reference's line."""
+    exe = os.path.join(ROOT, "helix_amd", "helix_reference_example")
+    if not os.path.exists(exe):
+        pytest.skip("helix_reference_example not built (the build had no reference tree next to it)")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "helix_cpp_example: steps=2 time=0.2 reduced_density_shape=(1,1024,1024)"
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_graph_chunks_match_eager_steps(hx, precision):
     """helix_hip_step replays 16-step hipGraph chunks; helix_hip_time_steps launches every stage eagerly.

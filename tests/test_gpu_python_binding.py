@@ -45,3 +45,32 @@ def test_spin_glass_against_oracle(helix, oracle_mod, tmp_path):
     path = tmp_path / "e.txt"
     assert helix.write_energy_trace(str(path), r)
     assert len(path.read_text().splitlines()) == 201
+
+
+def test_context_checkpoint_refuses_another_bath_or_time_step(helix, tmp_path):
+    """Context::load_checkpoint (C++ API) refuses a checkpoint written with another bath or time step, whose
+    hierarchy is the same, and resumes one of the same problem."""
+    def ctx(lam, dt):
+        co = helix.ContextOptions()
+        co.precision = helix.Precision.Double
+        co.timeStep = dt
+        h = helix.HierarchySpec()
+        h.maxDepth = 3
+        h.exponentialTerms = 2
+        h.baths = [helix.Bath.drude_lorentz(lam, 1.0, 1.0, 1) for _ in range(3)]
+        return helix.Context(co, helix.examples.spin_glass_system(3, 0), h)
+
+    path = str(tmp_path / "c.ckpt")
+    a = ctx(0.1, 0.01)
+    a.run_steps(5)
+    a.save_checkpoint(path)
+    a.destroy()
+    for lam, dt, what in ((0.2, 0.01, "coupling coefficients differ"), (0.1, 0.02, "time step")):
+        b = ctx(lam, dt)
+        with pytest.raises(ValueError, match=what):
+            b.load_checkpoint(path)
+        b.destroy()
+    c = ctx(0.1, 0.01)
+    c.load_checkpoint(path)
+    assert c.steps_taken() == 5
+    c.destroy()

@@ -41,7 +41,8 @@ def test_energy_trace_times_accumulate_like_the_cli(capi, tmp_path):
 
 def _info(capi, H=7, N=8, M=4):
     return capi.CheckpointInfo(n=N, n_ado=H, n_modes=M, n_baths=2, order=4, precision=0, steps=123,
-                               time=1.2300000000000002, dt=0.01, hierarchy_hash=0xDEADBEEF12345678)
+                               time=1.2300000000000002, dt=0.01, hierarchy_hash=0xDEADBEEF12345678,
+                               problem_hash=0x0123456789ABCDEF)
 
 
 def test_checkpoint_round_trip_exact(capi, tmp_path):
@@ -92,3 +93,17 @@ def test_hierarchy_hash_tells_hierarchies_apart(capi):
     c[[3, 4]] = c[[4, 3]]  # same rows, different order
     assert capi.hierarchy_hash(a) == capi.hierarchy_hash(a.copy())
     assert len({capi.hierarchy_hash(a), capi.hierarchy_hash(b), capi.hierarchy_hash(c)}) == 3
+
+
+def test_problem_hash_tells_problems_apart():
+    import helix_amd as hx
+    base = dict(spins=3, seed=1, lam=0.1, gamma=1.0, T=1.0, L=3, dt=0.01)
+    h0 = hx.Model.spin_glass(**base).problem_hash()
+    assert h0 != 0 and h0 == hx.Model.spin_glass(**base).problem_hash()
+    others = [dict(base, lam=0.2), dict(base, gamma=1.1), dict(base, T=0.9), dict(base, seed=2), dict(base, dt=0.02),
+              dict(base, h=0.9)]
+    hashes = {hx.Model.spin_glass(**o).problem_hash() for o in others}
+    assert len(hashes) == len(others) and h0 not in hashes
+    assert hx.Model.spin_glass(**base, order=3).problem_hash() != h0
+    # the storage precision does not enter: an FP64 checkpoint resumes an FP32 run of the same problem
+    assert hx.Model.spin_glass(**base, precision="fp32").problem_hash() == h0
