@@ -376,6 +376,23 @@ def main(argv=None):
                      "effective_full_block_GBs": full_per_launch / (mean_launch_ms * 1e-3) / 1e9},
         "launches_per_step": info["launches_per_step"],
     }
+    if args.dense_hamiltonian:
+        # the dense commutator is MFMA-bound: 8 N^3 flops per ADO and stage for a real H (H rho and rho H on the real
+        # and imaginary planes of rho), 16 N^3 for a complex H (the 64 N^3 per ADO-step of the full complex GEMMs)
+        hreal = bool(model.hamiltonian_is_real())
+        flops_launch = (8 if hreal else 16) * model.N ** 3 * model.H * (2 if args.precision == "split_fp32" else 1)
+        peak = 78.6 if args.precision == "fp64" else 157.3
+        tflops = flops_launch / (mean_launch_ms * 1e-3) / 1e12
+        line["roofline"] = {
+            "bound": "mfma", "achieved": tflops, "peak": peak, "unit": "TFLOP/s", "frac": tflops / peak,
+            "traffic": None,
+            "basis": f"{8 if hreal else 16} N^3 flops per ADO and stage issued on MFMA (real H: the imaginary-H "
+                     "products are skipped" + ("; split-FP32 issues them twice" if args.precision == "split_fp32" else "")
+                     + f"); complex-GEMM count 16 N^3 gives {16 * model.N ** 3 * model.H / (mean_launch_ms * 1e-3) / 1e12:.1f}",
+            "kernel": "k_stage_dense_tiled" if model.N > 64 else "k_stage_dense",
+            "flops_per_launch": flops_launch, "mean_launch_ms": mean_launch_ms,
+            "stage_ms": [float(x) for x in stage_ms],
+            "hbm_GBs_stored_bytes": achieved}
     traffic = None if args.dense_hamiltonian else _profiled_traffic(args.workload, args.precision, layout)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic

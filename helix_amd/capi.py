@@ -260,6 +260,15 @@ class Model:
     def problem_ptr(self):
         return host().helix_model_problem(self._h)
 
+    def hamiltonian_is_real(self) -> bool:
+        """True when every Hamiltonian entry the problem carries is real (dense, CSR or spin-structured form)."""
+        p = self.problem_ptr().contents
+        N = self.N
+        if p.hamiltonian_dense:
+            h = np.ctypeslib.as_array(p.hamiltonian_dense, shape=(2 * N * N,))
+            return bool(np.all(h[1::2] == 0.0))
+        return True
+
     def problem_hash(self) -> int:
         """helix_io_problem_hash of this model's problem descriptor."""
         return int(host().helix_io_problem_hash(C.cast(self.problem_ptr(), C.c_void_p)))

@@ -1340,6 +1340,250 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	}
 }
 
+// Dense-Hamiltonian stage for N = 128 .. 1024 (full row-major blocks): one 64 x 64 output tile of one ADO per
+// 256-thread workgroup, the tiles of one ADO on consecutive dispatch slots of one XCD (its block stays in that
+// L2).  G = H rho - rho H of the tile is accumulated on 16x16x4 MFMA over K = N in chunks of 16 staged in LDS:
+// the tile's 64 rows of H and of rho (A operands) and its 64 columns of rho and of H (B operands), each LDS row
+// padded by one element so that the 16 rows an operand read touches fall in different banks.  Wave w owns the
+// 32 x 32 quarter (w / 2, w % 2) as 2 x 2 MFMA tiles; the split-FP32 form adds the products with the FP32
+// remainder of H, a real H skips the imaginary-H products.  The epilogue runs on the accumulator registers:
+// register v of lane l in MFMA tile (ta, tb) holds element (I0 + 32 (w / 2) + 16 ta + row(l / 16, v),
+// J0 + 32 (w % 2) + 16 tb + l % 16); it adds (Sigma + Xi + closure) rho_n, the neighbour terms (elementwise in
+// the neighbour blocks) and applies the Taylor recurrence, as the whole-block kernel does.
+constexpr int kDT = 64;  // output tile edge
+constexpr int kDK = 16;  // K chunk staged per LDS round
+
+template <typename T, bool SPLIT, bool LAST, int LOGN, bool SZ>
+__global__ __launch_bounds__(kDenseThreads) void k_stage_dense_tiled(const StageArgs a, int ngroups)
+{
+	using V2 = typename C2t<T>::type;
+	using MF = Mfma<T>;
+	using acc4 = typename MF::acc;
+	constexpr int N = 1 << LOGN;
+	constexpr int LOGE = 2 * LOGN;
+	constexpr int NTT = N / kDT;
+	constexpr int TPA = NTT * NTT;
+	static_assert(N >= 2 * kDT && N % kDK == 0, "tiled dense path: N >= 128");
+	__shared__ V2 s_hr[kDT][kDK + 1];   // H[I0 + r][k0 + c]
+	__shared__ V2 s_rr[kDT][kDK + 1];   // rho[I0 + r][k0 + c]
+	__shared__ V2 s_rc[kDK][kDT + 1];   // rho[k0 + r][J0 + c]
+	__shared__ V2 s_hc[kDK][kDT + 1];   // H[k0 + r][J0 + c]
+	__shared__ V2 s_hrl[SPLIT ? kDT : 1][kDK + 1];
+	__shared__ V2 s_hcl[SPLIT ? kDK : 1][kDT + 1];
+	__shared__ double2 s_selfx[SZ ? N : 1];
+	__shared__ int s_nb[2 * kTileMaxModes];
+
+	const int per_xcd = (ngroups + kXcds - 1) / kXcds;
+	const int group = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+	if(group >= ngroups) return;
+	const int H = a.n_ado, M = a.M, B = a.B;
+	const int n = group / TPA;
+	const int tile = group - n * TPA;
+	const int I0 = (tile / NTT) * kDT, J0 = (tile % NTT) * kDT;
+	const int tid = threadIdx.x;
+	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
+	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);
+	const V2* __restrict__ hd = static_cast<const V2*>(a.hdense);
+	const V2* __restrict__ hl = static_cast<const V2*>(a.hdense_lo);
+
+	if constexpr(SZ)
+		for(int q = tid; q < N; q += kDenseThreads) s_selfx[q] = a.selfx[q];
+	for(int q = tid; q < 2 * M; q += kDenseThreads) {
+		const int v = (q < M) ? a.up[n * M + q] : a.down[n * M + (q - M)];
+		s_nb[q] = (v < H) ? v : -1;
+	}
+
+	const int wave = tid >> 6, lane = tid & 63;
+	const int li = lane & 15, lk = lane >> 4;
+	const int WR = (wave >> 1) * 32, WC = (wave & 1) * 32;
+	acc4 gr[2][2], gi[2][2];
+#pragma unroll
+	for(int ta = 0; ta < 2; ++ta)
+#pragma unroll
+		for(int tb = 0; tb < 2; ++tb) {
+			gr[ta][tb] = acc4{};
+			gi[ta][tb] = acc4{};
+		}
+	// staging: the A panels are 64 rows x 16 (thread: row tid / 4, four consecutive k), the B panels 16 rows x 64
+	// (thread: row tid / 16, four consecutive columns)
+	const int ar_ = tid >> 2, ac = (tid & 3) * 4;
+	const int br = tid >> 4, bc = (tid & 15) * 4;
+	for(int k0 = 0; k0 < N; k0 += kDK) {
+		__syncthreads();  // the previous chunk's operand reads are done
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			const size_t ea = static_cast<size_t>(I0 + ar_) * N + k0 + ac + u;
+			const size_t eb = static_cast<size_t>(k0 + br) * N + J0 + bc + u;
+			s_hr[ar_][ac + u] = hd[ea];
+			s_rr[ar_][ac + u] = own[ea];
+			s_rc[br][bc + u] = own[eb];
+			s_hc[br][bc + u] = hd[eb];
+			if constexpr(SPLIT) {
+				s_hrl[ar_][ac + u] = hl[ea];
+				s_hcl[br][bc + u] = hl[eb];
+			}
+		}
+		__syncthreads();
+#pragma unroll 1
+		for(int kk = 0; kk < kDK; kk += 4) {
+			const int k = kk + lk;
+			V2 ha[2], ra[2], hal[2], rb[2], hb[2], hbl[2];
+#pragma unroll
+			for(int t = 0; t < 2; ++t) {
+				ha[t] = s_hr[WR + 16 * t + li][k];
+				ra[t] = s_rr[WR + 16 * t + li][k];
+				rb[t] = s_rc[k][WC + 16 * t + li];
+				hb[t] = s_hc[k][WC + 16 * t + li];
+				if constexpr(SPLIT) {
+					hal[t] = s_hrl[WR + 16 * t + li][k];
+					hbl[t] = s_hcl[k][WC + 16 * t + li];
+				}
+			}
+#pragma unroll
+			for(int ta = 0; ta < 2; ++ta)
+#pragma unroll
+				for(int tb = 0; tb < 2; ++tb) {
+#pragma unroll
+					for(int part = 0; part < (SPLIT ? 2 : 1); ++part) {
+						const V2 h_a = part ? hal[ta] : ha[ta];
+						const V2 h_b = part ? hbl[tb] : hb[tb];
+						// H rho
+						gr[ta][tb] = MF::mma(h_a.x, rb[tb].x, gr[ta][tb]);
+						gi[ta][tb] = MF::mma(h_a.x, rb[tb].y, gi[ta][tb]);
+						// - rho H
+						gr[ta][tb] = MF::mma(-ra[ta].x, h_b.x, gr[ta][tb]);
+						gi[ta][tb] = MF::mma(-ra[ta].y, h_b.x, gi[ta][tb]);
+						if(!a.hreal) {
+							gr[ta][tb] = MF::mma(-h_a.y, rb[tb].y, gr[ta][tb]);
+							gi[ta][tb] = MF::mma(h_a.y, rb[tb].x, gi[ta][tb]);
+							gr[ta][tb] = MF::mma(ra[ta].y, h_b.y, gr[ta][tb]);
+							gi[ta][tb] = MF::mma(-ra[ta].x, h_b.y, gi[ta][tb]);
+						}
+					}
+				}
+		}
+	}
+
+	// the lane's 16 tile elements, four at a time (one MFMA tile): L = -i G + (Sigma + Xi + closure) rho_n, the
+	// neighbour terms, the Taylor recurrence
+	const double2 sf = a.selfc[n];
+	const int ks = a.kappa ? a.kslot[n] : -1;
+	const T sc = static_cast<T>(a.scale);
+	V2* __restrict__ out = static_cast<V2*>(a.out);
+#pragma unroll
+	for(int ta = 0; ta < 2; ++ta)
+#pragma unroll
+		for(int tb = 0; tb < 2; ++tb) {
+			T ar[4], ai[4];
+			int ei[4];
+#pragma unroll
+			for(int v = 0; v < 4; ++v) {
+				const int i = I0 + WR + 16 * ta + MF::row(lk, v), j = J0 + WC + 16 * tb + li;
+				const int e = (i << LOGN) | j;
+				ei[v] = e;
+				double wr = sf.x, wi = sf.y;
+				if constexpr(SZ) {
+					const double2 x = s_selfx[i ^ j];
+					wr += x.x;
+					wi += x.y;
+				} else {
+					for(int b = 0; b < B; ++b) {
+						const double vi = a.vdiag[b * N + i], vj = a.vdiag[b * N + j];
+						const double d = vi - vj, sm = vi + vj;
+						wr -= a.xi[b].x * d * d + a.xic[b].x * sm * d;
+						wi -= a.xi[b].y * d * d + a.xic[b].y * sm * d;
+					}
+				}
+				if(ks >= 0) {
+					const double2 kk = a.kappa[(static_cast<size_t>(ks) << LOGE) + e];
+					wr += kk.x;
+					wi += kk.y;
+				}
+				const V2 c = own[e];
+				T vr = gi[ta][tb][v], vim = -gr[ta][tb][v];
+				cmadd<T, SPLIT>(vr, vim, wr, wi, c.x, c.y);
+				ar[v] = vr;
+				ai[v] = vim;
+			}
+			for(int q = 0; q < 2 * M; ++q) {
+				const int idx = s_nb[q];
+				if(idx < 0) continue;
+				const bool down = q >= M;
+				const int k = down ? q - M : q;
+				const int b = a.mode_bath[k];
+				const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
+				V2 x[4];
+#pragma unroll
+				for(int v = 0; v < 4; ++v) x[v] = src[ei[v]];
+				if constexpr(SZ) {
+					const int mask = a.sz_mask[b];
+					const double two_alpha = 2.0 * a.sz_alpha[b];
+					double wcomm = -1.0, wanti = 0.0;
+					if(down) {
+						wcomm = a.dcomm[n * M + k].y;
+						wanti = a.danti[n * M + k].x;
+					}
+#pragma unroll
+					for(int v = 0; v < 4; ++v) {
+						const int bi = (ei[v] >> LOGN) & mask, bj = ei[v] & mask;
+						const bool differ = bi != bj;
+						const double sg = bi ? two_alpha : -two_alpha;
+						const double w = sg * (differ ? wcomm : wanti);
+						const T u = differ ? -x[v].y : x[v].x;
+						const T vv = differ ? x[v].x : x[v].y;
+						rmadd<T, SPLIT>(ar[v], ai[v], w, u, vv);
+					}
+				} else {
+					double2 wc = make_double2(0.0, -1.0), wa = make_double2(0.0, 0.0);
+					if(down) {
+						wc = a.dcomm[n * M + k];
+						wa = a.danti[n * M + k];
+					}
+					const double* __restrict__ vb = a.vdiag + b * N;
+#pragma unroll
+					for(int v = 0; v < 4; ++v) {
+						const double vi = vb[ei[v] >> LOGN], vj = vb[ei[v] & (N - 1)];
+						const double d = vi - vj, sm = vi + vj;
+						cmadd<T, SPLIT>(ar[v], ai[v], wc.x * d + wa.x * sm, wc.y * d + wa.y * sm, x[v].x, x[v].y);
+					}
+				}
+			}
+#pragma unroll
+			for(int v = 0; v < 4; ++v) {
+				const int e = ei[v];
+				const T tr = ar[v] * sc, ti = ai[v] * sc;
+				const size_t g = (static_cast<size_t>(n) << LOGE) + e;
+				if constexpr(!LAST) {
+					st_stream<T>(out + g, tr, ti);
+				} else {
+					if(a.rho0_f64 != nullptr && n == 0) {
+						double2* r0 = reinterpret_cast<double2*>(a.rho0_f64);
+						double2 rv = r0[e];
+						for(int q = 0; q < a.ntprev; ++q) {
+							const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
+							rv.x += static_cast<double>(t.x);
+							rv.y += static_cast<double>(t.y);
+						}
+						rv.x += static_cast<double>(tr);
+						rv.y += static_cast<double>(ti);
+						r0[e] = rv;
+						out[g] = V2{static_cast<T>(rv.x), static_cast<T>(rv.y)};
+					} else {
+						V2 rv = ld_stream<T>(static_cast<const V2*>(a.rho) + g);
+						for(int q = 0; q < a.ntprev; ++q) {
+							const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
+							rv.x += t.x;
+							rv.y += t.y;
+						}
+						rv.x += tr;
+						rv.y += ti;
+						st_stream<T>(out + g, rv.x, rv.y);
+					}
+				}
+			}
+		}
+}
+
 // Learned closure: feature rows for every (listed ADO a, element e = (i, j)):
 //   x = [n_b(i) for the logN spins (as +-1), n_b(j) likewise, extra[a][0 .. extra_dim)]
 __global__ __launch_bounds__(kBlock) void k_kappa_features(int count, int logN, const float* __restrict__ extra,
@@ -1420,7 +1664,7 @@ struct helix_hip_plan {
 	bool csr_h = false;       // general sparse H (CSR)
 	bool csr_v = false;       // general sparse couplings (CSR)
 	bool tiled = false;       // LDS-tiled stage kernel (diagonal couplings, N <= 64)
-	bool dense = false;       // dense Hamiltonian on MFMA (N in {16, 32, 64})
+	bool dense = false;       // dense Hamiltonian on MFMA (N = 16 .. 1024)
 	bool herm = false;        // Hermitian layout: N(N+1)/2 stored elements per ADO
 	int threads = 0;          // tile workgroup size (512 or 1024; N <= 64 sigma^z tiles are autotuned)
 	size_t esize = 16;        // bytes per complex element
@@ -1581,8 +1825,8 @@ int validate(const helix_hip_problem* pr)
 		return fail(HELIX_HIP_EINVAL, "unknown layout");
 	if(pr->hamiltonian_dense) {
 		if(pr->couplings_csr) return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path needs diagonal couplings");
-		if(pr->n != 16 && pr->n != 32 && pr->n != 64)
-			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports N = 16, 32, 64");
+		if(pr->n < 16 || pr->n > 1024)
+			return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path supports N = 16 .. 1024");
 		for(int b = 0; b < pr->n_baths; ++b)
 			if(pr->couplings[b].nflips != 0)
 				return fail(HELIX_HIP_EUNSUPPORTED, "dense Hamiltonian path needs diagonal couplings");
@@ -1681,11 +1925,23 @@ void launch_tile(helix_hip_plan* p, const StageArgs& a, bool last)
 template <typename T, bool SPLIT, int LOGN, bool SZ>
 void launch_dense_v(helix_hip_plan* p, const StageArgs& a, bool last)
 {
-	const unsigned grid = static_cast<unsigned>(((p->H + kXcds - 1) / kXcds) * kXcds);
-	if(last)
-		hipLaunchKernelGGL((k_stage_dense<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
-	else
-		hipLaunchKernelGGL((k_stage_dense<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+	if constexpr(LOGN <= 6) {
+		const unsigned grid = static_cast<unsigned>(((p->H + kXcds - 1) / kXcds) * kXcds);
+		if(last)
+			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+		else
+			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+	} else {
+		constexpr int NTT = (1 << LOGN) / kDT;
+		const int ngroups = p->H * NTT * NTT;
+		const unsigned grid = static_cast<unsigned>(((ngroups + kXcds - 1) / kXcds) * kXcds);
+		if(last)
+			hipLaunchKernelGGL((k_stage_dense_tiled<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream,
+				a, ngroups);
+		else
+			hipLaunchKernelGGL((k_stage_dense_tiled<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0,
+				p->stream, a, ngroups);
+	}
 }
 
 template <typename T, bool SPLIT, int LOGN>
@@ -1728,7 +1984,11 @@ void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 		switch(p->logN) {
 		case 4: launch_dense<T, SPLIT, 4>(p, a, last); return;
 		case 5: launch_dense<T, SPLIT, 5>(p, a, last); return;
-		default: launch_dense<T, SPLIT, 6>(p, a, last); return;
+		case 6: launch_dense<T, SPLIT, 6>(p, a, last); return;
+		case 7: launch_dense<T, SPLIT, 7>(p, a, last); return;
+		case 8: launch_dense<T, SPLIT, 8>(p, a, last); return;
+		case 9: launch_dense<T, SPLIT, 9>(p, a, last); return;
+		default: launch_dense<T, SPLIT, 10>(p, a, last); return;
 		}
 	}
 	if(p->tiled) {

@@ -95,9 +95,10 @@ def test_precision_form_n256_full_size(hx):
         plan.close()
 
 
-@pytest.mark.parametrize("spins,L", [(4, 3), (5, 2), (6, 2)])
+@pytest.mark.parametrize("spins,L", [(4, 3), (5, 2), (6, 2), (7, 1), (8, 1), (8, 2)])
 def test_rhs_parity_dense_hamiltonian_mfma_fp64(hx, oracle_mod, spins, L):
-    """Dense H on MFMA (N = 16, 32, 64) against the oracle's CSR commutator."""
+    """Dense H on MFMA against the oracle's CSR commutator: whole blocks (N = 16, 32, 64) and 64 x 64 output
+    tiles with K staged in LDS (N = 128, 256)."""
     m = hx.Model.spin_glass(spins, seed=5, L=L, lam=0.2, gamma=0.9, T=0.8).use_dense_hamiltonian()
     p = oracle_mod.spin_glass_problem(spins, 5, L=L, lam=0.2, gamma=0.9, T=0.8)
     plan = hx.Plan(m)
@@ -105,6 +106,32 @@ def test_rhs_parity_dense_hamiltonian_mfma_fp64(hx, oracle_mod, spins, L):
     s = _rand_state(m.H, m.N, 31)
     plan.set_state(s)
     assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+
+
+def test_dense_hamiltonian_precision_forms_n256(hx):
+    """The precision workload (8 spins, N = 256, 16 modes, tiers 0..3, 969 ADOs, dt 0.005) with -i[H, rho] on the
+    tiled MFMA kernel: FP64 equals the structured FP64 path to 1e-12 normwise, complex64 and split-FP32 stay within
+    1e-6 of it in rho_S, over 100 steps."""
+    kw = dict(spins=8, seed=2, L=3, dt=0.005)
+    ref = hx.Plan(hx.Model.spin_glass(**kw))
+    ref.set_initial()
+    ref.step(100)
+    r64 = ref.tier0()
+    s64 = ref.get_state()
+    ref.close()
+    for prec in ("fp64", "fp32", "split_fp32"):
+        m = hx.Model.spin_glass(**kw, precision=prec).use_dense_hamiltonian()
+        plan = hx.Plan(m)
+        assert plan.info()["dense"]
+        plan.set_initial()
+        plan.step(100)
+        if prec == "fp64":
+            assert _rel(plan.get_state(), s64) <= FP64_TOL
+        else:
+            err = float(np.max(np.abs(plan.tier0() - r64)))
+            print(f"dense N=256 {prec}: max|d rho_S| = {err:.3e}")
+            assert err <= FORM_TOL
+        plan.close()
 
 
 def test_dense_hamiltonian_trajectories(hx, oracle_mod):
