@@ -28,7 +28,7 @@ HIP_SYMBOLS = [
     "helix_hip_last_error", "helix_hip_batch_create", "helix_hip_batch_set_initial", "helix_hip_batch_step",
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
-    "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc",
+    "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc", "helix_hip_plan_set_terminator",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
@@ -57,6 +57,13 @@ class CheckpointInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("n_ado", C.c_int32), ("n_modes", C.c_int32), ("n_baths", C.c_int32),
                 ("order", C.c_int32), ("precision", C.c_int32), ("steps", C.c_uint64), ("time", C.c_double),
                 ("dt", C.c_double), ("hierarchy_hash", C.c_uint64), ("problem_hash", C.c_uint64)]
+
+
+class Terminator(C.Structure):
+    """Mirror of helix_hip_terminator."""
+    _fields_ = [("count", C.c_int32), ("rate", _dp), ("src_offsets", _ip), ("src_ado", _ip), ("src_mode", _ip),
+                ("src_comm", _dp), ("src_anti", _dp), ("n_up", C.c_int32), ("up_ado", _ip), ("up_mode", _ip),
+                ("up_virtual", _ip), ("h_vectors", _dp), ("h_values", _dp), ("iterations", C.c_int32)]
 
 
 class Problem(C.Structure):
@@ -98,6 +105,7 @@ def hip():
         L.helix_hip_rhs.argtypes = [vp, _dp]
         L.helix_hip_step.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
         L.helix_hip_step_noalloc.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
+        L.helix_hip_plan_set_terminator.argtypes = [vp, C.POINTER(Terminator)]
         L.helix_hip_get_tier0.argtypes = [vp, _dp]
         L.helix_hip_time_steps.argtypes = [vp, C.c_uint64, _dp, _dp]
         L.helix_hip_plan_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),

@@ -155,6 +155,122 @@ class Params:
                               axis=1).astype(np.float32)
 
 
+def hamiltonian_dense(model: Model) -> np.ndarray:
+    """H of a spin-structured model as a dense N x N array (diagonal plus bit flips)."""
+    t = model.tables()
+    N = model.N
+    h = np.diag(t["h_diag"]).astype(np.complex128)
+    for m, c in zip(t["h_masks"], t["h_coefs"]):
+        for i in range(N):
+            h[i, i ^ m] += c
+    return h
+
+
+class Terminator:
+    """The physics terminator of a hierarchy kept to tiers 0..L (include/helix_hip.h, helix_hip_plan_set_terminator):
+    the tier-(L+1) ADOs of the deeper hierarchy become virtual ADOs solved in every Taylor stage from the stationary
+    condition of their own equation.  Built from the tables of the hierarchy kept to tiers 0..L+1 (its tier-(L+1)
+    self rates and down coefficients) and the eigenbasis of H; `missing(x)` evaluates the same terms on the host."""
+
+    def __init__(self, params: Params, keep_tier: int, iterations: int = 3, dt: float = 0.01):
+        small = params.model(keep_tier, "fp64", dt)
+        big = params.model(keep_tier + 1, "fp64", dt)
+        self.N, self.H, self.M = small.N, small.H, small.M
+        rs, rb = small.rows(), big.rows()
+        tb = big.tables()
+        pos = {tuple(r): i for i, r in enumerate(rs)}
+        us = np.nonzero(rb.sum(1) == keep_tier + 1)[0]
+        upos = {int(u): q for q, u in enumerate(us)}
+        self.rate = np.ascontiguousarray(tb["self_coef"][us], np.complex128)
+        off, sado, smode, scomm, santi = [0], [], [], [], []
+        for u in us:
+            for l in range(big.M):
+                d = tb["down"][u, l]
+                if d < big.H:
+                    sado.append(pos[tuple(rb[d])])
+                    smode.append(l)
+                    scomm.append(tb["down_comm"][u, l])
+                    santi.append(tb["down_anti"][u, l])
+            off.append(len(sado))
+        self.src_offsets = np.array(off, np.int32)
+        self.src_ado, self.src_mode = np.array(sado, np.int32), np.array(smode, np.int32)
+        self.src_comm, self.src_anti = np.array(scomm, np.complex128), np.array(santi, np.complex128)
+        big_of = {tuple(r): i for i, r in enumerate(rb)}
+        uado, umode, uvirt = [], [], []
+        for n in np.nonzero(rs.sum(1) == keep_tier)[0]:
+            bn = big_of[tuple(rs[n])]
+            for k in range(big.M):
+                u = tb["up"][bn, k]
+                if u < big.H:
+                    uado.append(n)
+                    umode.append(k)
+                    uvirt.append(upos[int(u)])
+        self.up_ado, self.up_mode = np.array(uado, np.int32), np.array(umode, np.int32)
+        self.up_virtual = np.array(uvirt, np.int32)
+        self.evals, self.evecs = np.linalg.eigh(hamiltonian_dense(small))
+        self.evecs = np.ascontiguousarray(self.evecs, np.complex128)
+        self.iterations = iterations
+        ts = small.tables()
+        self.mode_bath = ts["mode_bath"]
+        spins = int(round(math.log2(self.N)))
+        sg = _sz_signs(spins)
+        self._dv = np.stack([sg[b][:, None] - sg[b][None, :] for b in self.mode_bath])
+        self._sv = np.stack([sg[b][:, None] + sg[b][None, :] for b in self.mode_bath])
+        B = len(ts["xi"])
+        dd = np.stack([sg[b][:, None] - sg[b][None, :] for b in range(B)])
+        self._selfw = -np.einsum("b,bij->ij", ts["xi"], dd * dd)
+        self.count = len(us)
+        small.close()
+        big.close()
+
+    def struct(self):
+        """helix_hip_terminator pointing into this object's arrays (keep the object alive while it is used)."""
+        self._r = self.rate.view(np.float64)
+        self._c = np.ascontiguousarray(self.src_comm).view(np.float64)
+        self._a = np.ascontiguousarray(self.src_anti).view(np.float64)
+        self._q = self.evecs.view(np.float64)
+        self._e = np.ascontiguousarray(self.evals, np.float64)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        return capi.Terminator(self.count, dp(self._r), ip(self.src_offsets), ip(self.src_ado), ip(self.src_mode),
+                               dp(self._c), dp(self._a), len(self.up_ado), ip(self.up_ado), ip(self.up_mode),
+                               ip(self.up_virtual), dp(self._q), dp(self._e), self.iterations)
+
+    def virtual(self, x: np.ndarray) -> np.ndarray:
+        """X_u of every virtual ADO from the kept state x [H][N][N] (the kernel's fixed-point rounds)."""
+        U, N = self.count, self.N
+        w = self.src_comm[:, None, None] * self._dv[self.src_mode] + self.src_anti[:, None, None] * self._sv[self.src_mode]
+        contrib = w * x[self.src_ado]
+        D = np.zeros((U, N, N), np.complex128)
+        seg = np.repeat(np.arange(U), np.diff(self.src_offsets))
+        np.add.at(D, seg, contrib)
+        Q = self.evecs
+        den = self.rate[:, None, None] - 1j * (self.evals[None, :, None] - self.evals[None, None, :])
+        X = np.zeros_like(D)
+        for _ in range(self.iterations):
+            R = D + self._selfw[None] * X
+            Rt = Q.conj().T[None] @ R @ Q[None]
+            X = Q[None] @ (-Rt / den) @ Q.conj().T[None]
+        return X
+
+    def missing(self, x: np.ndarray) -> np.ndarray:
+        """The terms the terminator adds, [H][N][N] (zero outside tier L)."""
+        X = self.virtual(x)
+        out = np.zeros_like(x)
+        np.add.at(out, self.up_ado, -1j * self._dv[self.up_mode] * X[self.up_virtual])
+        return out
+
+
+def set_terminator(plan: Plan, term: "Terminator | None"):
+    """Installs (or, with None, removes) the physics terminator on a plan (helix_hip_plan_set_terminator)."""
+    if term is None:
+        rc = hip().helix_hip_plan_set_terminator(plan._p, None)
+    else:
+        st = term.struct()
+        rc = hip().helix_hip_plan_set_terminator(plan._p, C.byref(st))
+    _check(rc, "helix_hip_plan_set_terminator")
+
+
 def _sz_signs(spins: int):
     N = 1 << spins
     s = np.array([[1.0 if (i >> b) & 1 else -1.0 for i in range(N)] for b in range(spins)])  # [b][i]

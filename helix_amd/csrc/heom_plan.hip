@@ -19,6 +19,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace helix_amd {
@@ -1584,6 +1585,107 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense_tiled(const Stage
 		}
 }
 
+// Physics terminator of the deepest kept tier (helix_hip_plan_set_terminator): virtual tier-(L+1) ADO u (block
+// n_real + u of the stage input) solved from the stationary condition of its own equation, in the eigenbasis of H
+// (include/helix_hip.h gives the equations).  One workgroup of N^2 threads per u, one element each; the four N x N
+// products per round go through LDS.  Arithmetic in FP64 whatever the state precision.
+struct TermArgs {
+	void* cur;                // stage input; virtual ADO u is block n_real + u
+	int n_real;
+	int iterations;
+	const double2* rate;      // [U]
+	const int32_t* src_off;   // [U+1]
+	const int32_t* src_ado;   // [S] stored index of the source
+	const int32_t* src_mask;  // [S] bit mask of the spin of the edge's bath
+	const double* src_alpha;  // [S] alpha of that bath (V_b = alpha (2 n_s - 1))
+	const double2* src_comm;  // [S]
+	const double2* src_anti;  // [S]
+	const double2* q;         // [N][N] eigenvectors of H (columns)
+	const double* ev;         // [N] eigenvalues
+	const double2* selfx;     // [N] the plan's sigma^z Xi self table, indexed by i XOR j
+};
+
+template <typename T, int LOGN>
+__global__ __launch_bounds__(1 << (2 * LOGN)) void k_terminator(const TermArgs t)
+{
+	using V2 = typename C2t<T>::type;
+	constexpr int N = 1 << LOGN;
+	constexpr int E = N * N;
+	__shared__ double2 s_q[E];
+	__shared__ double2 s_a[E];
+	__shared__ double s_ev[N];
+	const int e = threadIdx.x;
+	const int i = e >> LOGN, j = e & (N - 1);
+	const int u = blockIdx.x;
+	s_q[e] = t.q[e];
+	if(e < N) s_ev[e] = t.ev[e];
+	V2* __restrict__ cur = static_cast<V2*>(t.cur);
+	// D_u[ij] = sum_s (comm (v_i - v_j) + anti (v_i + v_j)) rho_src[ij]
+	double dr = 0.0, di = 0.0;
+	for(int s = t.src_off[u]; s < t.src_off[u + 1]; ++s) {
+		const V2 x = cur[static_cast<size_t>(t.src_ado[s]) * E + e];
+		const int m = t.src_mask[s];
+		const double al = t.src_alpha[s];
+		const double vi = (i & m) ? al : -al, vj = (j & m) ? al : -al;
+		const double d = vi - vj, sm = vi + vj;
+		const double2 c = t.src_comm[s], an = t.src_anti[s];
+		const double wr = c.x * d + an.x * sm, wi = c.y * d + an.y * sm;
+		const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
+		dr += wr * xr - wi * xi;
+		di += wr * xi + wi * xr;
+	}
+	const double2 rate = t.rate[u];
+	const double2 sw = t.selfx[i ^ j];
+	__syncthreads();  // s_q, s_ev
+	double xr = 0.0, xi = 0.0;
+	for(int it = 0; it < t.iterations; ++it) {
+		// R = D + S X_prev, then R Q
+		s_a[e] = make_double2(dr + sw.x * xr - sw.y * xi, di + sw.x * xi + sw.y * xr);
+		__syncthreads();
+		double ar = 0.0, ai = 0.0;
+		for(int k = 0; k < N; ++k) {
+			const double2 r = s_a[i * N + k], qv = s_q[k * N + j];
+			ar += r.x * qv.x - r.y * qv.y;
+			ai += r.x * qv.y + r.y * qv.x;
+		}
+		__syncthreads();
+		s_a[e] = make_double2(ar, ai);
+		__syncthreads();
+		// R~ = Q^dagger (R Q), element (a, b) = (i, j); X~ = -R~ / (rate - i (E_a - E_b))
+		double tr = 0.0, ti = 0.0;
+		for(int k = 0; k < N; ++k) {
+			const double2 qv = s_q[k * N + i], av = s_a[k * N + j];
+			tr += qv.x * av.x + qv.y * av.y;
+			ti += qv.x * av.y - qv.y * av.x;
+		}
+		const double den_r = rate.x, den_i = rate.y - (s_ev[i] - s_ev[j]);
+		const double inv = 1.0 / (den_r * den_r + den_i * den_i);
+		const double yr = -(tr * den_r + ti * den_i) * inv, yi = -(ti * den_r - tr * den_i) * inv;
+		__syncthreads();
+		// X = Q X~ Q^dagger
+		s_a[e] = make_double2(yr, yi);
+		__syncthreads();
+		double br = 0.0, bi = 0.0;
+		for(int k = 0; k < N; ++k) {
+			const double2 y = s_a[i * N + k], qv = s_q[j * N + k];
+			br += y.x * qv.x + y.y * qv.y;
+			bi += y.y * qv.x - y.x * qv.y;
+		}
+		__syncthreads();
+		s_a[e] = make_double2(br, bi);
+		__syncthreads();
+		xr = 0.0;
+		xi = 0.0;
+		for(int k = 0; k < N; ++k) {
+			const double2 qv = s_q[i * N + k], b = s_a[k * N + j];
+			xr += qv.x * b.x - qv.y * b.y;
+			xi += qv.x * b.y + qv.y * b.x;
+		}
+		__syncthreads();
+	}
+	cur[static_cast<size_t>(t.n_real + u) * E + e] = V2{static_cast<T>(xr), static_cast<T>(xi)};
+}
+
 // Learned closure: feature rows for every (listed ADO a, element e = (i, j)):
 //   x = [n_b(i) for the logN spins (as +-1), n_b(j) likewise, extra[a][0 .. extra_dim)]
 __global__ __launch_bounds__(kBlock) void k_kappa_features(int count, int logN, const float* __restrict__ extra,
@@ -1692,6 +1794,19 @@ struct helix_hip_plan {
 	size_t trace_doubles = 0;
 	// this device's copy of `scratch` (helix_hip_step_noalloc)
 	double* scratch = nullptr;
+	// physics terminator (helix_hip_plan_set_terminator): U virtual ADOs after the H real ones in every state vector
+	// (capacity virt_cap), the tables extended to H + U rows while it is on, the original tables kept
+	int term_count = 0;
+	int virt_cap = 0;
+	TermArgs term{};
+	const int32_t* orig_up = nullptr;
+	const int32_t* orig_down = nullptr;
+	const double2* orig_dcomm = nullptr;
+	const double2* orig_danti = nullptr;
+	const double2* orig_selfc = nullptr;
+	std::vector<void*> term_allocs;  // the extended tables and terminator arguments of the current terminator
+	std::vector<int32_t> kslot_host;  // closure slot of every stored ADO (-1: none), H + virt_cap entries on the device
+	std::vector<int32_t> mode_bath_host;
 };
 
 namespace helix_amd_mlp {
@@ -2012,8 +2127,27 @@ void launch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 	}
 }
 
+template <typename T>
+void launch_terminator(helix_hip_plan* p, const void* cur)
+{
+	TermArgs t = p->term;
+	t.cur = const_cast<void*>(cur);
+	const dim3 grid(static_cast<unsigned>(p->term_count));
+	switch(p->logN) {
+	case 3: hipLaunchKernelGGL((k_terminator<T, 3>), grid, dim3(64), 0, p->stream, t); break;
+	case 4: hipLaunchKernelGGL((k_terminator<T, 4>), grid, dim3(256), 0, p->stream, t); break;
+	default: hipLaunchKernelGGL((k_terminator<T, 5>), grid, dim3(1024), 0, p->stream, t); break;
+	}
+}
+
 void dispatch_stage(helix_hip_plan* p, const StageArgs& a, bool last)
 {
+	if(p->term_count > 0) {
+		if(p->precision == HELIX_HIP_FP64)
+			launch_terminator<double>(p, a.cur);
+		else
+			launch_terminator<float>(p, a.cur);
+	}
 	switch(p->precision) {
 	case HELIX_HIP_FP64: launch_stage<double, false>(p, a, last); break;
 	case HELIX_HIP_FP32: launch_stage<float, false>(p, a, last); break;
@@ -2547,6 +2681,7 @@ int helix_hip_plan_create(const helix_hip_problem* pr, int device, helix_hip_pla
 	std::memset(&a, 0, sizeof(a));
 	const size_t HM = static_cast<size_t>(p->H) * p->M;
 	p->perm = colex_order(pr->up, p->H, p->M);
+	p->mode_bath_host.assign(pr->mode_bath, pr->mode_bath + p->M);
 	const StoredTables tabs = permute_tables(pr, p->perm);
 	if((rc = upload(p, tabs.up.data(), HM, &a.up)) || (rc = upload(p, tabs.down.data(), HM, &a.down)) ||
 		(rc = upload(p, pr->mode_bath, static_cast<size_t>(p->M), &a.mode_bath)))
@@ -2686,6 +2821,7 @@ void helix_hip_plan_destroy(helix_hip_plan* p)
 	drop_chunk_graph(p);
 	for(hipEvent_t e : p->events) (void)hipEventDestroy(e);
 	for(void* q : p->allocations) (void)hipFree(q);
+	for(void* q : p->term_allocs) (void)hipFree(q);
 	if(p->trace) (void)hipFree(p->trace);
 	if(p->stream) (void)hipStreamDestroy(p->stream);
 	delete p;
@@ -2812,7 +2948,7 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 	int din = 0, dout = 0;
 	helix_amd_mlp::dims(mlp, &din, &dout);
 	if(din != 2 * p->logN + extra_dim || dout != 2) return fail(HELIX_HIP_EINVAL, "MLP sizes must be (2 log2 N + extra, 2)");
-	std::vector<int32_t> slot(p->H, -1);
+	std::vector<int32_t> slot(static_cast<size_t>(p->H) + p->virt_cap, -1);
 	for(int a = 0; a < count; ++a) {
 		if(ados[a] < 0 || ados[a] >= p->H) return fail(HELIX_HIP_EINVAL, "closure ADO index out of range");
 		slot[p->perm[ados[a]]] = a;
@@ -2822,13 +2958,14 @@ int helix_hip_plan_set_closure(helix_hip_plan* p, helix_hip_mlp* mlp, const int3
 	int rc = 0;
 	void* q = nullptr;
 	if(!p->kslot || p->kappa_count < count) {
-		if((rc = plan_alloc(p, &q, sizeof(int32_t) * p->H))) return rc;
+		if((rc = plan_alloc(p, &q, sizeof(int32_t) * (static_cast<size_t>(p->H) + p->virt_cap)))) return rc;
 		p->kslot = static_cast<int32_t*>(q);
 		if((rc = plan_alloc(p, &q, sizeof(double2) * rows))) return rc;
 		p->kappa = static_cast<double2*>(q);
 		p->kappa_count = count;
 	}
-	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * p->H, hipMemcpyHostToDevice));
+	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice));
+	p->kslot_host = slot;
 	float* dextra = nullptr;
 	if(extra_dim > 0) {
 		HX_HIP(hipMalloc(&dextra, sizeof(float) * count * extra_dim));
@@ -2859,7 +2996,7 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 	}
 	if(!ados || count <= 0) return fail(HELIX_HIP_EINVAL, "bad closure rates");
 	if(p->herm) return fail(HELIX_HIP_EUNSUPPORTED, "closure rates need a plan in HELIX_HIP_LAYOUT_FULL");
-	std::vector<int32_t> slot(p->H, -1);
+	std::vector<int32_t> slot(static_cast<size_t>(p->H) + p->virt_cap, -1);
 	for(int a = 0; a < count; ++a) {
 		if(ados[a] < 0 || ados[a] >= p->H) return fail(HELIX_HIP_EINVAL, "closure ADO index out of range");
 		slot[p->perm[ados[a]]] = a;
@@ -2868,16 +3005,169 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* p, const int32_t* ados, int
 	int rc = 0;
 	void* q = nullptr;
 	if(!p->kslot || p->kappa_count < count) {
-		if((rc = plan_alloc(p, &q, sizeof(int32_t) * p->H))) return rc;
+		if((rc = plan_alloc(p, &q, sizeof(int32_t) * (static_cast<size_t>(p->H) + p->virt_cap)))) return rc;
 		p->kslot = static_cast<int32_t*>(q);
 		if((rc = plan_alloc(p, &q, sizeof(double2) * rows))) return rc;
 		p->kappa = static_cast<double2*>(q);
 		p->kappa_count = count;
 	}
-	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * p->H, hipMemcpyHostToDevice));
+	HX_HIP(hipMemcpy(p->kslot, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice));
+	p->kslot_host = slot;
 	HX_HIP(hipMemcpy(p->kappa, kappa, sizeof(double2) * rows, hipMemcpyHostToDevice));
 	p->base.kappa = p->kappa;
 	p->base.kslot = p->kslot;
+	return p->order >= 2 ? capture_chunk(p) : 0;
+}
+
+int helix_hip_plan_set_terminator(helix_hip_plan* p, const helix_hip_terminator* tm)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	drop_chunk_graph(p);
+	StageArgs& a = p->base;
+	if(!p->orig_up) {
+		p->orig_up = a.up;
+		p->orig_down = a.down;
+		p->orig_dcomm = a.dcomm;
+		p->orig_danti = a.danti;
+		p->orig_selfc = a.selfc;
+	}
+	auto free_term = [&]() {
+		for(void* q : p->term_allocs) (void)hipFree(q);
+		p->term_allocs.clear();
+	};
+	if(!tm) {
+		a.up = p->orig_up;
+		a.down = p->orig_down;
+		a.dcomm = p->orig_dcomm;
+		a.danti = p->orig_danti;
+		a.selfc = p->orig_selfc;
+		a.n_ado = p->H;
+		p->term_count = 0;
+		free_term();
+		return p->order >= 2 ? capture_chunk(p) : 0;
+	}
+	const int U = tm->count, N = p->N, H = p->H, M = p->M;
+	if(U <= 0 || !tm->rate || !tm->src_offsets || !tm->h_vectors || !tm->h_values || tm->n_up < 0 ||
+		(tm->n_up > 0 && (!tm->up_ado || !tm->up_mode || !tm->up_virtual)) || tm->iterations < 1 || tm->iterations > 16)
+		return fail(HELIX_HIP_EINVAL, "bad terminator");
+	if(p->herm || !a.sz || !p->tiled || p->logN > 5)
+		return fail(HELIX_HIP_EUNSUPPORTED, "the terminator needs a full-layout plan with sigma^z couplings and N <= 32");
+	const int S = tm->src_offsets[U];
+	if(tm->src_offsets[0] != 0 || S < 0 || (S > 0 && (!tm->src_ado || !tm->src_mode || !tm->src_comm || !tm->src_anti)))
+		return fail(HELIX_HIP_EINVAL, "bad terminator sources");
+	for(int u = 0; u < U; ++u)
+		if(tm->src_offsets[u + 1] < tm->src_offsets[u]) return fail(HELIX_HIP_EINVAL, "terminator source offsets decrease");
+	for(int q = 0; q < S; ++q)
+		if(tm->src_ado[q] < 0 || tm->src_ado[q] >= H || tm->src_mode[q] < 0 || tm->src_mode[q] >= M)
+			return fail(HELIX_HIP_EINVAL, "terminator source out of range");
+	for(int q = 0; q < tm->n_up; ++q)
+		if(tm->up_ado[q] < 0 || tm->up_ado[q] >= H || tm->up_mode[q] < 0 || tm->up_mode[q] >= M || tm->up_virtual[q] < 0 ||
+			tm->up_virtual[q] >= U)
+			return fail(HELIX_HIP_EINVAL, "terminator up entry out of range");
+	int rc = 0;
+	// state vectors with room for the virtual ADOs (rho keeps its H real blocks)
+	if(U > p->virt_cap) {
+		const size_t blocks = static_cast<size_t>(H) + U;
+		const size_t bytes = blocks * p->pe * p->esize;
+		auto replace = [&](void*& buf, bool keep) -> int {
+			void* nb = nullptr;
+			if(int r = plan_alloc(p, &nb, bytes)) return r;
+			HX_HIP(hipMemset(nb, 0, bytes));
+			if(keep && buf) HX_HIP(hipMemcpy(nb, buf, p->elems * p->esize, hipMemcpyDeviceToDevice));
+			if(buf) {
+				auto it = std::find(p->allocations.begin(), p->allocations.end(), buf);
+				if(it != p->allocations.end()) p->allocations.erase(it);
+				(void)hipFree(buf);
+				p->device_bytes -= p->elems * p->esize + static_cast<size_t>(p->virt_cap) * p->pe * p->esize;
+			}
+			buf = nb;
+			return 0;
+		};
+		if((rc = replace(p->rho, true))) return rc;
+		for(int q = 1; q <= std::max(1, p->order - 1); ++q)
+			if((rc = replace(p->t[q], false))) return rc;
+		p->virt_cap = U;
+		if(p->kslot) {  // the closure slots of the virtual rows
+			std::vector<int32_t> slot = p->kslot_host;
+			slot.resize(static_cast<size_t>(H) + U, -1);
+			void* q = nullptr;
+			if((rc = plan_alloc(p, &q, sizeof(int32_t) * slot.size()))) return rc;
+			HX_HIP(hipMemcpy(q, slot.data(), sizeof(int32_t) * slot.size(), hipMemcpyHostToDevice));
+			p->kslot = static_cast<int32_t*>(q);
+			p->kslot_host = slot;
+			if(a.kslot) a.kslot = p->kslot;
+		}
+	}
+	// tables extended by the U virtual rows: no neighbours, no coefficients; absent entries point past them
+	const size_t HM = static_cast<size_t>(H) * M, HUM = (static_cast<size_t>(H) + U) * M;
+	std::vector<int32_t> up(HUM, H + U), down(HUM, H + U);
+	std::vector<double2> dcomm(HUM, make_double2(0.0, 0.0)), danti(HUM, make_double2(0.0, 0.0));
+	std::vector<double2> selfc(static_cast<size_t>(H) + U, make_double2(0.0, 0.0));
+	HX_HIP(hipMemcpy(up.data(), p->orig_up, HM * sizeof(int32_t), hipMemcpyDeviceToHost));
+	HX_HIP(hipMemcpy(down.data(), p->orig_down, HM * sizeof(int32_t), hipMemcpyDeviceToHost));
+	HX_HIP(hipMemcpy(dcomm.data(), p->orig_dcomm, HM * sizeof(double2), hipMemcpyDeviceToHost));
+	HX_HIP(hipMemcpy(danti.data(), p->orig_danti, HM * sizeof(double2), hipMemcpyDeviceToHost));
+	HX_HIP(hipMemcpy(selfc.data(), p->orig_selfc, static_cast<size_t>(H) * sizeof(double2), hipMemcpyDeviceToHost));
+	for(size_t q = 0; q < HM; ++q) {
+		if(up[q] >= H) up[q] = H + U;
+		if(down[q] >= H) down[q] = H + U;
+	}
+	for(int q = 0; q < tm->n_up; ++q) {
+		const size_t slot = static_cast<size_t>(p->perm[tm->up_ado[q]]) * M + tm->up_mode[q];
+		if(up[slot] != H + U) return fail(HELIX_HIP_EINVAL, "terminator up entry where the ADO has an up-neighbour");
+		up[slot] = H + tm->up_virtual[q];
+	}
+	// terminator arguments
+	std::vector<int32_t> sado(std::max(S, 1)), smask(std::max(S, 1));
+	std::vector<double> salpha(std::max(S, 1));
+	for(int q = 0; q < S; ++q) {
+		const int b = p->mode_bath_host[tm->src_mode[q]];
+		sado[q] = p->perm[tm->src_ado[q]];
+		smask[q] = a.sz_mask[b];
+		salpha[q] = a.sz_alpha[b];
+	}
+	free_term();
+	auto up_t = [&](const auto* host, size_t count, auto** dev) -> int {
+		using X = std::remove_const_t<std::remove_pointer_t<decltype(host)>>;
+		void* d = nullptr;
+		if(count == 0) count = 1;
+		HX_HIP(hipMalloc(&d, count * sizeof(X)));
+		p->term_allocs.push_back(d);
+		HX_HIP(hipMemcpy(d, host, count * sizeof(X), hipMemcpyHostToDevice));
+		*dev = static_cast<const X*>(d);
+		return 0;
+	};
+	const int32_t *d_up = nullptr, *d_down = nullptr;
+	const double2 *d_dcomm = nullptr, *d_danti = nullptr, *d_selfc = nullptr;
+	TermArgs& t = p->term;
+	t = TermArgs{};
+	const int32_t* d_off = nullptr;
+	const double* d_ev = nullptr;
+	if((rc = up_t(up.data(), HUM, &d_up)) || (rc = up_t(down.data(), HUM, &d_down)) || (rc = up_t(dcomm.data(), HUM, &d_dcomm)) ||
+		(rc = up_t(danti.data(), HUM, &d_danti)) || (rc = up_t(selfc.data(), selfc.size(), &d_selfc)) ||
+		(rc = up_t(reinterpret_cast<const double2*>(tm->rate), static_cast<size_t>(U), &t.rate)) ||
+		(rc = up_t(tm->src_offsets, static_cast<size_t>(U) + 1, &d_off)) || (rc = up_t(sado.data(), sado.size(), &t.src_ado)) ||
+		(rc = up_t(smask.data(), smask.size(), &t.src_mask)) || (rc = up_t(salpha.data(), salpha.size(), &t.src_alpha)) ||
+		(rc = up_t(reinterpret_cast<const double2*>(S ? tm->src_comm : tm->rate), std::max<size_t>(S, 1), &t.src_comm)) ||
+		(rc = up_t(reinterpret_cast<const double2*>(S ? tm->src_anti : tm->rate), std::max<size_t>(S, 1), &t.src_anti)) ||
+		(rc = up_t(reinterpret_cast<const double2*>(tm->h_vectors), static_cast<size_t>(N) * N, &t.q)) ||
+		(rc = up_t(tm->h_values, static_cast<size_t>(N), &d_ev))) {
+		free_term();
+		return rc;
+	}
+	t.src_off = d_off;
+	t.ev = d_ev;
+	t.n_real = H;
+	t.iterations = tm->iterations;
+	t.selfx = a.selfx;
+	a.up = d_up;
+	a.down = d_down;
+	a.dcomm = d_dcomm;
+	a.danti = d_danti;
+	a.selfc = d_selfc;
+	a.n_ado = H + U;
+	p->term_count = U;
 	return p->order >= 2 ? capture_chunk(p) : 0;
 }
 

@@ -183,6 +183,35 @@ int helix_hip_plan_set_closure_rates(helix_hip_plan* plan, const int32_t* ados, 
 int helix_hip_plan_set_closure(helix_hip_plan* plan, struct helix_hip_mlp* mlp, const int32_t* ados, int count,
 	const float* extra, int extra_dim);
 
+/* Physics terminator of the deepest kept tier L.  No counterpart in the reference, whose deepest tier simply loses
+ * its up-neighbour terms (src/liouville.cu:818-823).  In every Taylor stage each of the `count` virtual tier-(L+1)
+ * ADOs u is solved from the stationary condition of its own equation, its own up-neighbours dropped:
+ *   0 = -i[H, X_u] + (rate[u] + S) X_u + sum_s ( comm[s] [V_b(s), rho_src[s]] + anti[s] {V_b(s), rho_src[s]} )
+ * (s over the sources of u: its tier-L down-neighbours, b(s) the bath of the edge's mode; S the plan's elementwise
+ * Xi self term of sigma^z couplings).  The solve works in the eigenbasis of H = Q diag(E) Q^dagger,
+ *   X~_ab = -(Q^dagger (D + S X_prev) Q)_ab / (rate[u] - i (E_a - E_b)),
+ * `iterations` fixed-point rounds for S (X_prev = 0 first; S does not commute with H).  Every listed kept tier-L
+ * ADO up_ado[e] then gains (-i)[V_b(up_mode[e]), X_up_virtual[e]], the term a real up-neighbour would give.  Linear
+ * in the state, so exact inside every Taylor stage; combines with the closure rates above.  Needs a plan in
+ * HELIX_HIP_LAYOUT_FULL with sigma^z couplings, N <= 32, on the tiled stage kernel.  NULL switches it off. */
+typedef struct helix_hip_terminator {
+	int32_t count;               /* virtual ADOs U */
+	const double* rate;          /* [U][2] self coefficient of each virtual ADO (-sum_k nu_k n_k) */
+	const int32_t* src_offsets;  /* [U+1] sources of u: src_offsets[u] .. src_offsets[u+1]-1 */
+	const int32_t* src_ado;      /* [S] kept tier-L ADO (hierarchy order) */
+	const int32_t* src_mode;     /* [S] mode k of the edge (u = src + e_k) */
+	const double* src_comm;      /* [S][2] down_comm of u for mode k */
+	const double* src_anti;      /* [S][2] down_anti of u for mode k */
+	int32_t n_up;
+	const int32_t* up_ado;       /* [n_up] kept tier-L ADO n (hierarchy order) */
+	const int32_t* up_mode;      /* [n_up] mode k with n + e_k = virtual ADO up_virtual */
+	const int32_t* up_virtual;   /* [n_up] u */
+	const double* h_vectors;     /* [N][N][2] unitary Q whose columns are eigenvectors of H */
+	const double* h_values;      /* [N] eigenvalues E */
+	int32_t iterations;          /* fixed-point rounds for S, 1..16 */
+} helix_hip_terminator;
+int helix_hip_plan_set_terminator(helix_hip_plan* plan, const helix_hip_terminator* terminator);
+
 /* ---- batched form: many independent problems advanced by one launch per Taylor stage ----
  * No counterpart in the reference (one hard-coded problem per process, src/helix.cpp:73-104).
  * Every problem must have N <= 64, sigma^z couplings, a spin-structured H, and the same precision
