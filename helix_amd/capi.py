@@ -29,6 +29,7 @@ HIP_SYMBOLS = [
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
     "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc", "helix_hip_plan_set_terminator",
+    "helix_hip_plan_set_terminator_gain", "helix_hip_get_ados", "helix_hip_batch_get_ados",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
@@ -107,6 +108,8 @@ def hip():
         L.helix_hip_step_noalloc.argtypes = [vp, C.c_uint64, C.c_uint64, _dp, _dp]
         L.helix_hip_plan_set_terminator.argtypes = [vp, C.POINTER(Terminator)]
         L.helix_hip_get_tier0.argtypes = [vp, _dp]
+        L.helix_hip_get_ados.argtypes = [vp, _ip, C.c_int, _dp]
+        L.helix_hip_plan_set_terminator_gain.argtypes = [vp, _dp]
         L.helix_hip_time_steps.argtypes = [vp, C.c_uint64, _dp, _dp]
         L.helix_hip_plan_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                           C.POINTER(C.c_int)]
@@ -118,6 +121,7 @@ def hip():
         L.helix_hip_batch_step.argtypes = [vp, C.c_uint64]
         L.helix_hip_batch_get_tier0.argtypes = [vp, C.c_int, _dp]
         L.helix_hip_batch_get_state.argtypes = [vp, C.c_int, _dp]
+        L.helix_hip_batch_get_ados.argtypes = [vp, C.c_int, _ip, C.c_int, _dp]
         L.helix_hip_batch_time_steps.argtypes = [vp, C.c_uint64, _dp]
         L.helix_hip_batch_info.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_size_t), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int)]
@@ -410,6 +414,14 @@ class Plan:
         _check(hip().helix_hip_get_tier0(self._p, _pd(r.view(np.float64))), "helix_hip_get_tier0")
         return r
 
+    def ados(self, ados) -> np.ndarray:
+        """Blocks of the listed ADOs (hierarchy order) as [len(ados)][N][N] complex128 (helix_hip_get_ados)."""
+        idx = np.ascontiguousarray(ados, np.int32)
+        out = np.zeros((idx.size, self.N, self.N), np.complex128)
+        _check(hip().helix_hip_get_ados(self._p, idx.ctypes.data_as(_ip), idx.size, _pd(out.view(np.float64))),
+               "helix_hip_get_ados")
+        return out
+
     def time_steps(self, nsteps: int):
         st = np.zeros(self.model.order)
         tot = C.c_double()
@@ -472,6 +484,15 @@ class Batch:
         s = np.zeros((m.H, m.N, m.N), np.complex128)
         _check(hip().helix_hip_batch_get_state(self._b, index, _pd(s.view(np.float64))), "batch_get_state")
         return s
+
+    def ados(self, index: int, ados) -> np.ndarray:
+        """Blocks of the listed ADOs of problem `index` as [len(ados)][N][N] complex128 (helix_hip_batch_get_ados)."""
+        N = self.models[index].N
+        idx = np.ascontiguousarray(ados, np.int32)
+        out = np.zeros((idx.size, N, N), np.complex128)
+        _check(hip().helix_hip_batch_get_ados(self._b, index, idx.ctypes.data_as(_ip), idx.size,
+                                              _pd(out.view(np.float64))), "helix_hip_batch_get_ados")
+        return out
 
     def time_steps(self, nsteps: int) -> float:
         ms = C.c_double()

@@ -1,18 +1,25 @@
-"""Learned closure of a truncated hierarchy.
+"""Closures of a truncated hierarchy: the physics terminator and its learned correction.
 
 A hierarchy kept to tiers 0..L loses, for every tier-L ADO n, the up-neighbour term
     c_n = -i sum_k [V_{b(k)}, rho_{n+e_k}]
-(the reference's plain truncation reads a zero block there, src/liouville.cu:818-823).  For sigma^z
-couplings a Markovian terminator replaces it by an elementwise rate, c_n[ij] ~ kappa_n[ij] rho_n[ij];
-here kappa is learned: a 3-layer perceptron (include/helix_mlp.h: BF16 weights, FP32 accumulation,
-trained on the GPU by its own HIP kernels) maps the spin configuration of the element (n_b(i), n_b(j)),
-the ADO's multi-index and the model parameters (lambda, gamma, T, J_ij) to (Re kappa, Im kappa).
-Training targets come from the plain FP64 solver at a deeper truncation (the batched GPU form), where
-the tier-(L+1) ADOs exist: per parameter set, ADO and element, the least-squares rate over the recorded
-snapshots, kappa* = sum_t conj(rho_t) c_t / sum_t |rho_t|^2.  The plan evaluates the network once
-(helix_hip_plan_set_closure) and its stage kernels apply kappa in every Taylor stage; a calibration
-window against a deeper plain run decides whether the closure is trusted or the solver falls back to
-plain truncation.
+(the reference's plain truncation reads a zero block there, src/liouville.cu:818-823).
+
+Physics terminator (include/helix_hip.h, helix_hip_plan_set_terminator): every tier-(L+1) ADO u of the deeper
+hierarchy becomes a virtual ADO X_u solved in each Taylor stage from the stationary condition of its own
+equation, its up-neighbours dropped.  In the eigenbasis of H = Q diag(E) Q^dagger that is
+    X~_ab = -R~_ab / (r_u - i (E_a - E_b)),   R = D_u + S X_u,   D_u = sum over u's tier-L down-neighbours,
+a linear map of the tier-L ADOs.
+
+Learned correction (helix_hip_plan_set_terminator_gain): the response of each eigen-element is multiplied by
+c_u[ab] = 1 + MLP(lambda, gamma, T, J_ij, multi-index of u, its rate r_u, E_a, E_b, E_a - E_b), a 2 x 256 tanh
+perceptron with BF16 weights whose GEMMs run on BF16 MFMA with FP32 accumulation (include/helix_mlp.h), trained
+on the GPU by its own HIP kernels.  The closure stays linear in the tier-L ADOs (exact inside every Taylor stage)
+and maps them to the missing tier-(L+1) contribution.  Targets: along trajectories of the plain FP64 solver at
+a deep truncation (tiers 0..6, the batched GPU form), where the tier-(L+1) ADOs exist, the least-squares
+response of every (u, a, b), c*_u[ab] = -(sum_t conj(R~_t) X~_t / sum_t |R~_t|^2) (r_u - i (E_a - E_b)),
+weighted by sum_t |R~_t|^2.  ClosureSolver runs the kept hierarchy with plain truncation, the physics
+terminator or the learned one, and falls back to plain truncation when a calibration run against the deep
+plain solver exceeds a tolerance.
 """
 from __future__ import annotations
 
@@ -146,14 +153,6 @@ class Params:
         return Model.spin_glass(self.spins, self.seed, lam=self.lam, gamma=self.gamma, T=self.T, K=1, L=max_tier,
                                 dt=dt, precision=precision)
 
-    def extra(self, rows: np.ndarray, max_tier: int) -> np.ndarray:
-        """Per-ADO extra features: multi-index / L, lambda, gamma, T, J_ij (i < j)."""
-        J = capi.spin_glass_couplings(self.spins, self.seed)
-        iu = np.triu_indices(self.spins, 1)
-        glob = np.concatenate([[self.lam, self.gamma, self.T], J[iu]])
-        return np.concatenate([rows.astype(np.float64) / max_tier, np.tile(glob, (rows.shape[0], 1))],
-                              axis=1).astype(np.float32)
-
 
 def hamiltonian_dense(model: Model) -> np.ndarray:
     """H of a spin-structured model as a dense N x N array (diagonal plus bit flips)."""
@@ -181,6 +180,8 @@ class Terminator:
         pos = {tuple(r): i for i, r in enumerate(rs)}
         us = np.nonzero(rb.sum(1) == keep_tier + 1)[0]
         upos = {int(u): q for q, u in enumerate(us)}
+        self.params, self.keep_tier = params, keep_tier
+        self.virtual_rows = np.ascontiguousarray(rb[us])  # multi-index of every virtual ADO u
         self.rate = np.ascontiguousarray(tb["self_coef"][us], np.complex128)
         off, sado, smode, scomm, santi = [0], [], [], [], []
         for u in us:
@@ -277,134 +278,205 @@ def _sz_signs(spins: int):
     return s
 
 
-def missing_terms(model: Model, state: np.ndarray, tier: int):
-    """Tier-`tier` ADO indices of a deeper state and their terms c_n = -i sum_k [V_{b(k)}, rho_{n+e_k}]."""
-    t = model.tables()
-    rows = model.rows()
-    spins = int(round(math.log2(model.N)))
-    sg = _sz_signs(spins)
-    idx = np.nonzero(rows.sum(1) == tier)[0]
-    c = np.zeros((len(idx), model.N, model.N), np.complex128)
-    for k in range(model.M):
-        u = t["up"][idx, k]
-        ok = u < model.H
-        if not np.any(ok):
-            continue
-        b = t["mode_bath"][k]
-        d = sg[b][:, None] - sg[b][None, :]
-        c[ok] += -1j * d[None] * state[u[ok]]
-    return idx, c
 
 
-def element_bits(spins: int) -> np.ndarray:
-    """[N*N][2*spins]: +-1 spin configuration of row i then column j for every element e = i*N + j."""
-    N = 1 << spins
-    e = np.arange(N * N)
-    i, j = e // N, e % N
-    bi = np.stack([np.where((i >> b) & 1, 1.0, -1.0) for b in range(spins)], 1)
-    bj = np.stack([np.where((j >> b) & 1, 1.0, -1.0) for b in range(spins)], 1)
-    return np.concatenate([bi, bj], 1).astype(np.float32)
+# ---------------------------------------------------------------- learned correction of the terminator
+
+# fixed input scaling of the network's features (the parameter ranges of Params.draw)
+_FEATURE_SCALE = dict(lam=0.5, gamma=2.0, T=2.0, rate=10.0, energy=4.0)
 
 
-def training_pairs(params_list, keep_tier: int = 2, deep_tier: int = 6, steps: int = 2000, every: int = 100,
-                   dt: float = 0.01, min_weight: float = 1e-3, max_rate: float = 5.0):
-    """Runs the plain FP64 solver at `deep_tier` for every parameter set (one batched launch per stage),
-    records every `every`-th state, and returns (features, kappa*) rows, one per (set, tier-`keep_tier`
-    ADO, element) whose block element was non-negligible over the window."""
-    models = [p.model(deep_tier, "fp64", dt) for p in params_list]
-    batch = capi.Batch(models, graph=True, autotune=False)
-    batch.set_initial()
-    num = [None] * len(models)
-    den = [None] * len(models)
-    for snap in range(steps // every):
-        batch.step(every)
-        for q, m in enumerate(models):
-            state = batch.state(q)
-            idx, c = missing_terms(m, state, keep_tier)
-            rho = state[idx]
-            if num[q] is None:
-                num[q] = np.zeros_like(c)
-                den[q] = np.zeros(c.shape)
-            num[q] += np.conj(rho) * c
-            den[q] += np.abs(rho) ** 2
-    batch.close()
+def correction_features(term: Terminator) -> np.ndarray:
+    """[U * N * N][din] float32 feature rows of every virtual ADO u and eigen-element (a, b), row u * N^2 + a N + b:
+    lambda, gamma, T, J_ij (i < j), the multi-index of u / (L + 1), Re and Im of its rate r_u, E_a, E_b, E_a - E_b."""
+    p, U, N = term.params, term.count, term.N
+    J = capi.spin_glass_couplings(p.spins, p.seed)
+    glob = np.concatenate([[p.lam / _FEATURE_SCALE["lam"], p.gamma / _FEATURE_SCALE["gamma"], p.T / _FEATURE_SCALE["T"]],
+                           J[np.triu_indices(p.spins, 1)]])
+    per_u = np.concatenate([term.virtual_rows / (term.keep_tier + 1.0),
+                            np.stack([term.rate.real, term.rate.imag], 1) / _FEATURE_SCALE["rate"]], 1)
+    e = term.evals / _FEATURE_SCALE["energy"]
+    ea, eb = np.meshgrid(e, e, indexing="ij")
+    pair = np.stack([ea.ravel(), eb.ravel(), (ea - eb).ravel()], 1)
+    rows = np.concatenate([np.repeat(np.tile(glob, (U, 1)), N * N, 0), np.repeat(per_u, N * N, 0),
+                           np.tile(pair, (U, 1))], 1)
+    return np.ascontiguousarray(rows, np.float32)
+
+
+def feature_dim(spins: int, keep_tier: int = 2) -> int:
+    """Input width of the correction network for `spins` spins (Pade K = 1: 2 modes per spin)."""
+    return 3 + spins * (spins - 1) // 2 + 2 * spins + 2 + 3
+
+
+def physical_response(term: Terminator) -> np.ndarray:
+    """[U][N][N] denominators r_u - i (E_a - E_b) of the physics terminator."""
+    return term.rate[:, None, None] - 1j * (term.evals[None, :, None] - term.evals[None, None, :])
+
+
+def predicted_correction(term: Terminator, mlp: "MLP") -> np.ndarray:
+    """[U][N][N] complex c = 1 + MLP(features), evaluated on the GPU."""
+    y = mlp.forward(correction_features(term)).astype(np.float64)
+    return (1.0 + y[:, 0] + 1j * y[:, 1]).reshape(term.count, term.N, term.N)
+
+
+def set_terminator_gain(plan: Plan, corr: "np.ndarray | None"):
+    """Multiplies the installed terminator's response by corr [U][N][N] (helix_hip_plan_set_terminator_gain)."""
+    if corr is None:
+        rc = hip().helix_hip_plan_set_terminator_gain(plan._p, None)
+    else:
+        c = np.ascontiguousarray(corr, np.complex128)
+        rc = hip().helix_hip_plan_set_terminator_gain(plan._p, c.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)))
+    _check(rc, "helix_hip_plan_set_terminator_gain")
+
+
+class _DeepIndex:
+    """Positions, in the deep hierarchy (tiers 0..deep), of a terminator's tier-L sources and virtual ADOs."""
+
+    def __init__(self, term: Terminator, deep_rows: np.ndarray, small_rows: np.ndarray):
+        pos = {tuple(r): i for i, r in enumerate(deep_rows)}
+        self.src_small = np.unique(term.src_ado)
+        self.deep_src = np.array([pos[tuple(small_rows[a])] for a in self.src_small], np.int32)
+        self.deep_virtual = np.array([pos[tuple(r)] for r in term.virtual_rows], np.int32)
+        local = {int(a): q for q, a in enumerate(self.src_small)}
+        self.src_local = np.array([local[int(a)] for a in term.src_ado], np.int32)
+
+
+def response_terms(W, Qv, selfw, xs, X, src_local, seg):
+    """One snapshot's contribution to the response statistics of a group of sets: W [S][Q][N][N] source weights,
+    Qv [S][N][N] eigenvectors of H, selfw [S][N][N] Xi self table, xs [S][sources][N][N] tier-L ADOs, X [S][U][N][N]
+    tier-(L+1) ADOs, src_local [Q] source of each edge, seg [U] first edge of each u.  Returns conj(R~) X~ and
+    |R~|^2, R = D + S X, D_u = sum of u's edges W x_src, ~ the eigenbasis of H."""
+    Qh = np.conj(np.transpose(Qv, (0, 2, 1)))
+    D = np.add.reduceat(W * xs[:, src_local], seg, axis=1)
+    R = D + selfw[:, None] * X
+    Rt = Qh[:, None] @ R @ Qv[:, None]
+    Xt = Qh[:, None] @ X @ Qv[:, None]
+    return np.conj(Rt) * Xt, np.abs(Rt) ** 2
+
+
+def source_weights(term: Terminator) -> np.ndarray:
+    """[Q][N][N] elementwise weight of each source edge: comm (v_i - v_j) + anti (v_i + v_j)."""
+    return term.src_comm[:, None, None] * term._dv[term.src_mode] + term.src_anti[:, None, None] * term._sv[term.src_mode]
+
+
+def response_statistics(params_list, keep_tier: int = 2, deep_tier: int = 6, steps: int = 2000, every: int = 100,
+                        dt: float = 0.01, chunk: int = 1024, log=None):
+    """Runs the plain FP64 solver at `deep_tier` for every parameter set (batched GPU form, `chunk` sets per batch),
+    and accumulates along the trajectory, for every virtual ADO u of the tier-`keep_tier` terminator and
+    eigen-element (a, b): num = sum_t conj(R~) X~ and den = sum_t |R~|^2, X the deep run's tier-(L+1) ADO and
+    R = D_u + S X.  Returns (terminators, num [S][U][N][N], den [S][U][N][N])."""
+    terms = [Terminator(p, keep_tier) for p in params_list]
+    U, N = terms[0].count, terms[0].N
+    small_rows = params_list[0].model(keep_tier, "fp64", dt).rows()
+    deep0 = params_list[0].model(deep_tier, "fp64", dt)
+    idx = _DeepIndex(terms[0], deep0.rows(), small_rows)
+    deep0.close()
+    want = np.concatenate([idx.deep_src, idx.deep_virtual]).astype(np.int32)
+    ns = len(idx.src_small)
+    seg = np.asarray(terms[0].src_offsets[:-1], np.int64)
+    num = np.zeros((len(params_list), U, N, N), np.complex128)
+    den = np.zeros((len(params_list), U, N, N))
+    for c0 in range(0, len(params_list), chunk):
+        sets = list(range(c0, min(c0 + chunk, len(params_list))))
+        models = [params_list[q].model(deep_tier, "fp64", dt) for q in sets]
+        batch = capi.Batch(models, graph=True, autotune=False)
+        batch.set_initial()
+        # per set: source weights [Q][N][N], eigenvectors, Xi self table
+        W = np.stack([source_weights(terms[q]) for q in sets])
+        Qv = np.stack([terms[q].evecs for q in sets])
+        selfw = np.stack([terms[q]._selfw for q in sets])
+        for snap in range(steps // every):
+            batch.step(every)
+            blocks = np.stack([batch.ados(k, want) for k in range(len(sets))])
+            a, b = response_terms(W, Qv, selfw, blocks[:, :ns], blocks[:, ns:], idx.src_local, seg)
+            num[sets] += a
+            den[sets] += b
+            if log:
+                log(f"sets {sets[0]}..{sets[-1]}: step {(snap + 1) * every}/{steps}")
+        batch.close()
+        for m in models:
+            m.close()
+    return terms, num, den
+
+
+def correction_targets(terms, num, den, rel_weight: float = 1e-3, clip: float = 3.0, per_set: int = 4096, seed: int = 0):
+    """Training rows (features, [Re, Im] of c* - 1) from response statistics: per set the eigen-elements whose
+    weight den is at least rel_weight x the set's largest, with |c* - 1| <= clip, at most per_set of them chosen
+    at random with probability proportional to their weight."""
+    rng = np.random.default_rng(seed)
     X, Y = [], []
-    for q, (p, m) in enumerate(zip(params_list, models)):
-        rows = m.rows()
-        idx = np.nonzero(rows.sum(1) == keep_tier)[0]
-        extra = p.extra(rows[idx], keep_tier)
-        bits = element_bits(p.spins)
-        nn = num[q].reshape(len(idx), -1)
-        dd = den[q].reshape(len(idx), -1)
-        # elements that carry weight in the dynamics of their ADO (sum_t |rho_t|^2 >= min_weight x the largest)
-        keep = dd > min_weight * dd.max(axis=1, keepdims=True)
-        kappa = np.where(keep, nn / np.where(keep, dd, 1.0), 0.0)
-        keep &= np.abs(kappa) < max_rate
-        for a in range(len(idx)):
-            sel = np.nonzero(keep[a])[0]
-            if sel.size == 0:
-                continue
-            X.append(np.concatenate([bits[sel], np.tile(extra[a], (sel.size, 1))], 1))
-            Y.append(np.stack([kappa[a, sel].real, kappa[a, sel].imag], 1))
+    for q, term in enumerate(terms):
+        d = den[q].ravel()
+        c = (-(num[q] / np.maximum(den[q], 1e-300)) * physical_response(term)).ravel() - 1.0
+        ok = np.nonzero((d >= rel_weight * d.max()) & (np.abs(c) <= clip))[0]
+        if ok.size > per_set:
+            ok = rng.choice(ok, per_set, replace=False, p=d[ok] / d[ok].sum())
+        feats = correction_features(term)
+        X.append(feats[ok])
+        Y.append(np.stack([c[ok].real, c[ok].imag], 1))
     return np.concatenate(X).astype(np.float32), np.concatenate(Y).astype(np.float32)
 
 
-class LearnedClosureSolver:
-    """Truncated hierarchy (tiers 0..keep_tier, complex64 by default) plus the learned closure, with a
-    calibration window against a deeper plain run that falls back to plain truncation when the closure's
-    error exceeds `tolerance`."""
+def save_network(path: str, mlp: "MLP", **meta):
+    """Writes the trained network (float32 master parameters, dims) and metadata to an .npz file."""
+    np.savez(path, params=mlp.params(), dims=np.array(mlp.dims, np.int64),
+             **{k: np.asarray(v) for k, v in meta.items()})
 
-    def __init__(self, params: Params, mlp: MLP, keep_tier: int = 2, precision: str = "fp32", dt: float = 0.01):
-        self.params, self.mlp, self.keep_tier, self.dt = params, mlp, keep_tier, dt
+
+def load_network(path: str, device: int = 0) -> "MLP":
+    """The network saved by save_network, on the GPU."""
+    with np.load(path, allow_pickle=False) as f:
+        dims = [int(v) for v in f["dims"]]
+        mlp = MLP(*dims, device=device)
+        mlp.set_params(f["params"])
+    return mlp
+
+
+class ClosureSolver:
+    """A hierarchy kept to tiers 0..keep_tier, advanced with plain truncation ("plain"), the physics terminator
+    ("physics") or the learned correction of it ("learned").  calibrate() runs a window against the plain FP64
+    solver at a deep truncation and falls back to plain truncation when the closure's max|d rho_S| exceeds the
+    tolerance."""
+
+    def __init__(self, params: Params, mlp: "MLP | None" = None, keep_tier: int = 2, precision: str = "fp32",
+                 dt: float = 0.01, iterations: int = 5):
+        self.params, self.keep_tier, self.dt = params, keep_tier, dt
         self.model = params.model(keep_tier, precision, dt)
-        self.plan = Plan(self.model, layout="full")  # the rates need not be conjugate-symmetric
-        rows = self.model.rows()
-        self.ados = np.ascontiguousarray(np.nonzero(rows.sum(1) == keep_tier)[0], np.int32)
-        self.extra = np.ascontiguousarray(params.extra(rows[self.ados], keep_tier), np.float32)
-        assert mlp.dims[0] == 2 * params.spins + self.extra.shape[1] and mlp.dims[3] == 2
-        self.closure_on = False
-        self.calibration_error = None
+        self.plan = Plan(self.model, layout="full")
+        self.term = Terminator(params, keep_tier, iterations=iterations, dt=dt)
+        self.corr = predicted_correction(self.term, mlp) if mlp is not None else None
+        self.mode = "plain"
+        self.calibration = None
 
-    def enable(self, on: bool = True):
-        L = _bind()
-        if on:
-            rc = L.helix_hip_plan_set_closure(self.plan._p, self.mlp.handle,
-                                              self.ados.ctypes.data_as(C.POINTER(C.c_int32)), len(self.ados),
-                                              _f(self.extra), self.extra.shape[1])
-        else:
-            rc = L.helix_hip_plan_set_closure(self.plan._p, None, None, 0, None, 0)
-        _check(rc, "helix_hip_plan_set_closure")
-        self.closure_on = on
+    def use(self, mode: str):
+        if mode not in ("plain", "physics", "learned"):
+            raise ValueError(mode)
+        if mode == "learned" and self.corr is None:
+            raise HelixError("no network given: the learned closure is unavailable")
+        set_terminator(self.plan, None if mode == "plain" else self.term)
+        if mode == "learned":
+            set_terminator_gain(self.plan, self.corr)
+        self.mode = mode
 
-    def calibrate(self, window: int = 500, deep_tier: int = 6, tolerance: float | None = None) -> bool:
-        """Runs `window` steps of the plain solver at `deep_tier` (the reference), of plain truncation and of
-        the closure.  The closure is kept only when it is closer to the reference than plain truncation
-        and, if `tolerance` is given, within max|d rho_S| <= tolerance; otherwise the solver falls back to
-        plain truncation."""
+    def run(self, steps: int, record_every: int = 0):
+        """rho_S after `steps` steps from rho_0 (and its trace every record_every steps when given)."""
+        self.plan.set_initial()
+        trace, _ = self.plan.step(steps, record_every, record_rho=record_every > 0)
+        return (self.plan.tier0(), trace) if record_every else self.plan.tier0()
+
+    def calibrate(self, mode: str = "learned", window: int = 500, deep_tier: int = 6, tolerance: float = 1e-2) -> bool:
+        """Runs `window` steps of the deep plain FP64 solver and of the closure; keeps the closure when its
+        max|d rho_S| over the window is within tolerance, else falls back to plain truncation."""
         deep = Plan(self.params.model(deep_tier, "fp64", self.dt))
         deep.set_initial()
-        deep.step(window)
-        ref = deep.tier0()
+        ref, _ = deep.step(window, 10, record_rho=True)
         deep.close()
-        self.enable(False)
-        self.plan.set_initial()
-        self.plan.step(window)
-        self.plain_calibration_error = float(np.max(np.abs(self.plan.tier0() - ref)))
-        self.enable(True)
-        self.plan.set_initial()
-        self.plan.step(window)
-        self.calibration_error = float(np.max(np.abs(self.plan.tier0() - ref)))
-        trusted = self.calibration_error < self.plain_calibration_error
-        if tolerance is not None:
-            trusted = trusted and self.calibration_error <= tolerance
-        self.enable(trusted)
-        self.plan.set_initial()
-        return trusted
-
-    def run(self, steps: int) -> np.ndarray:
-        self.plan.set_initial()
-        self.plan.step(steps)
-        return self.plan.tier0()
+        self.use(mode)
+        _, tr = self.run(window, 10)
+        err = float(np.max(np.abs(tr - ref)))
+        self.calibration = dict(mode=mode, window=window, error=err, tolerance=tolerance, kept=err <= tolerance)
+        self.use(mode if err <= tolerance else "plain")
+        return err <= tolerance
 
     def close(self):
         self.plan.close()

@@ -1603,6 +1603,7 @@ struct TermArgs {
 	const double2* q;         // [N][N] eigenvectors of H (columns)
 	const double* ev;         // [N] eigenvalues
 	const double2* selfx;     // [N] the plan's sigma^z Xi self table, indexed by i XOR j
+	const double2* corr;      // [U][N][N] complex factor on each eigen-element's response, or null (factor 1)
 };
 
 template <typename T, int LOGN>
@@ -1660,7 +1661,13 @@ __global__ __launch_bounds__(1 << (2 * LOGN)) void k_terminator(const TermArgs t
 		}
 		const double den_r = rate.x, den_i = rate.y - (s_ev[i] - s_ev[j]);
 		const double inv = 1.0 / (den_r * den_r + den_i * den_i);
-		const double yr = -(tr * den_r + ti * den_i) * inv, yi = -(ti * den_r - tr * den_i) * inv;
+		double yr = -(tr * den_r + ti * den_i) * inv, yi = -(ti * den_r - tr * den_i) * inv;
+		if(t.corr) {  // learned correction of the response of eigen-element (a, b) = (i, j)
+			const double2 c = t.corr[static_cast<size_t>(u) * E + e];
+			const double zr = yr * c.x - yi * c.y;
+			yi = yr * c.y + yi * c.x;
+			yr = zr;
+		}
 		__syncthreads();
 		// X = Q X~ Q^dagger
 		s_a[e] = make_double2(yr, yi);
@@ -1749,6 +1756,18 @@ __global__ __launch_bounds__(kBlock) void k_record(const void* tier0, const doub
 
 using namespace helix_amd;
 
+// Copies `count` stored ADO blocks of `words` 8-byte words each, block q from base + offs[q] bytes, into out
+// back to back (helix_hip_get_ados, helix_hip_batch_get_ados).
+__global__ __launch_bounds__(kBlock) void k_gather_blocks(const char* __restrict__ base, const int64_t* __restrict__ offs,
+	int count, int words, uint2* __restrict__ out)
+{
+	const size_t total = static_cast<size_t>(count) * words;
+	for(size_t t = blockIdx.x * static_cast<size_t>(kBlock) + threadIdx.x; t < total; t += gridDim.x * static_cast<size_t>(kBlock)) {
+		const size_t q = t / words, w = t - q * words;
+		out[t] = reinterpret_cast<const uint2*>(base + offs[q])[w];
+	}
+}
+
 // Device scratch of helix_hip_step_noalloc (include/helix_hip.h): the staging buffer of its trace records.  Each GPU
 // the library runs on holds its own copy of this array until the process ends; host code never passes the
 // array's name to a kernel or a copy (on the host it only stands in for the symbol), it uses the device address
@@ -1805,6 +1824,7 @@ struct helix_hip_plan {
 	const double2* orig_danti = nullptr;
 	const double2* orig_selfc = nullptr;
 	std::vector<void*> term_allocs;  // the extended tables and terminator arguments of the current terminator
+	void* term_corr = nullptr;       // its learned correction table (one of term_allocs), or null
 	std::vector<int32_t> kslot_host;  // closure slot of every stored ADO (-1: none), H + virt_cap entries on the device
 	std::vector<int32_t> mode_bath_host;
 };
@@ -2609,6 +2629,46 @@ int state_to_host(helix_hip_plan* p, const void* dev, double* out, bool tier0_ma
 	return 0;
 }
 
+// owns a device buffer for the length of one host call
+struct DeviceBuffer {
+	void* ptr = nullptr;
+	~DeviceBuffer()
+	{
+		if(ptr) (void)hipFree(ptr);
+	}
+};
+
+// Stored blocks at the given byte offsets from base (pe elements of esize bytes each) gathered on the device and
+// downloaded as FP64 interleaved, block after block.
+int gather_stored(hipStream_t stream, const void* base, const std::vector<int64_t>& offs, size_t pe, size_t esize,
+	std::vector<double>& out)
+{
+	const int count = static_cast<int>(offs.size());
+	const size_t block_bytes = pe * esize;
+	const size_t off_bytes = (offs.size() * sizeof(int64_t) + 255) & ~size_t(255);
+	DeviceBuffer d;
+	HX_HIP(hipMalloc(&d.ptr, off_bytes + count * block_bytes));
+	char* dev = static_cast<char*>(d.ptr);
+	HX_HIP(hipMemcpyAsync(dev, offs.data(), offs.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+	const int words = static_cast<int>(block_bytes / 8);
+	const size_t total = static_cast<size_t>(count) * words;
+	const unsigned grid = static_cast<unsigned>(std::min<size_t>((total + kBlock - 1) / kBlock, 4096));
+	hipLaunchKernelGGL(k_gather_blocks, dim3(grid), dim3(kBlock), 0, stream, static_cast<const char*>(base),
+		reinterpret_cast<const int64_t*>(dev), count, words, reinterpret_cast<uint2*>(dev + off_bytes));
+	if(int rc = launch_error("gather blocks")) return rc;
+	out.resize(static_cast<size_t>(count) * pe * 2);
+	if(esize == 16) {
+		HX_HIP(hipMemcpyAsync(out.data(), dev + off_bytes, count * block_bytes, hipMemcpyDeviceToHost, stream));
+		HX_HIP(hipStreamSynchronize(stream));
+	} else {
+		std::vector<float> f(static_cast<size_t>(count) * pe * 2);
+		HX_HIP(hipMemcpyAsync(f.data(), dev + off_bytes, count * block_bytes, hipMemcpyDeviceToHost, stream));
+		HX_HIP(hipStreamSynchronize(stream));
+		for(size_t q = 0; q < f.size(); ++q) out[q] = static_cast<double>(f[q]);
+	}
+	return 0;
+}
+
 // problem coefficients under which L maps Hermitian ADOs to Hermitian ADOs (see helix_hip_layout)
 bool preserves_hermiticity(const helix_hip_problem* pr)
 {
@@ -2895,6 +2955,28 @@ int helix_hip_get_tier0(helix_hip_plan* p, double* out)
 	return 0;
 }
 
+int helix_hip_get_ados(helix_hip_plan* p, const int32_t* ados, int count, double* out)
+{
+	if(!p || !ados || !out || count <= 0) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	std::vector<int64_t> offs(static_cast<size_t>(count));
+	for(int q = 0; q < count; ++q) {
+		if(ados[q] < 0 || ados[q] >= p->H) return fail(HELIX_HIP_EINVAL, "ADO index out of range");
+		const int64_t s = p->perm.empty() ? ados[q] : p->perm[ados[q]];
+		offs[q] = s * static_cast<int64_t>(p->pe * p->esize);
+	}
+	HX_HIP(hipStreamSynchronize(p->stream));
+	std::vector<double> st;
+	if(int rc = gather_stored(p->stream, p->rho, offs, p->pe, p->esize, st)) return rc;
+	const size_t n2 = static_cast<size_t>(p->N) * p->N;
+	for(int q = 0; q < count; ++q) {
+		double* blk = st.data() + static_cast<size_t>(q) * p->pe * 2;
+		if(ados[q] == 0 && p->precision != HELIX_HIP_FP64)  // the FP64 master of the tier-0 block
+			HX_HIP(hipMemcpy(blk, p->rho0_f64, p->pe * 16, hipMemcpyDeviceToHost));
+		from_stored(blk, 1, p->logN, p->herm, {}, out + static_cast<size_t>(q) * n2 * 2);
+	}
+	return 0;
+}
+
 int helix_hip_time_steps(helix_hip_plan* p, uint64_t nsteps, double* stage_ms, double* total_ms)
 {
 	if(!p || nsteps == 0) return fail(HELIX_HIP_EINVAL, "bad arguments");
@@ -3035,6 +3117,7 @@ int helix_hip_plan_set_terminator(helix_hip_plan* p, const helix_hip_terminator*
 	auto free_term = [&]() {
 		for(void* q : p->term_allocs) (void)hipFree(q);
 		p->term_allocs.clear();
+		p->term_corr = nullptr;
 	};
 	if(!tm) {
 		a.up = p->orig_up;
@@ -3168,6 +3251,31 @@ int helix_hip_plan_set_terminator(helix_hip_plan* p, const helix_hip_terminator*
 	a.selfc = d_selfc;
 	a.n_ado = H + U;
 	p->term_count = U;
+	return p->order >= 2 ? capture_chunk(p) : 0;
+}
+
+int helix_hip_plan_set_terminator_gain(helix_hip_plan* p, const double* corr)
+{
+	if(!p) return fail(HELIX_HIP_EINVAL, "plan is null");
+	if(p->term_count == 0) return fail(HELIX_HIP_EINVAL, "no terminator is installed on this plan");
+	HX_HIP(hipStreamSynchronize(p->stream));
+	drop_chunk_graph(p);
+	if(p->term_corr) {
+		auto it = std::find(p->term_allocs.begin(), p->term_allocs.end(), p->term_corr);
+		if(it != p->term_allocs.end()) p->term_allocs.erase(it);
+		(void)hipFree(p->term_corr);
+		p->term_corr = nullptr;
+	}
+	p->term.corr = nullptr;
+	if(corr) {
+		const size_t bytes = static_cast<size_t>(p->term_count) * p->N * p->N * sizeof(double2);
+		void* d = nullptr;
+		HX_HIP(hipMalloc(&d, bytes));
+		p->term_allocs.push_back(d);
+		p->term_corr = d;
+		HX_HIP(hipMemcpy(d, corr, bytes, hipMemcpyHostToDevice));
+		p->term.corr = static_cast<const double2*>(d);
+	}
 	return p->order >= 2 ? capture_chunk(p) : 0;
 }
 
@@ -3657,6 +3765,29 @@ int helix_hip_batch_get_state(helix_hip_batch* b, int index, double* state)
 	if(b->precision != HELIX_HIP_FP64)
 		HX_HIP(hipMemcpy(st.data(), b->rho0_f64 + 2 * b->r0_off[index], pe * 16, hipMemcpyDeviceToHost));
 	from_stored(st.data(), b->H[index], b->logN[index], b->herm, b->perm[index], state);
+	return 0;
+}
+
+int helix_hip_batch_get_ados(helix_hip_batch* b, int index, const int32_t* ados, int count, double* out)
+{
+	if(!b || !ados || !out || count <= 0 || index < 0 || index >= b->Q) return fail(HELIX_HIP_EINVAL, "bad arguments");
+	const size_t pe = b->pe[index];
+	std::vector<int64_t> offs(static_cast<size_t>(count));
+	for(int q = 0; q < count; ++q) {
+		if(ados[q] < 0 || ados[q] >= b->H[index]) return fail(HELIX_HIP_EINVAL, "ADO index out of range");
+		offs[q] = static_cast<int64_t>(b->perm[index][ados[q]]) * static_cast<int64_t>(pe * b->esize);
+	}
+	HX_HIP(hipStreamSynchronize(b->stream));
+	const char* base = static_cast<const char*>(b->rho) + b->elem_off[index] * b->esize;
+	std::vector<double> st;
+	if(int rc = gather_stored(b->stream, base, offs, pe, b->esize, st)) return rc;
+	const size_t n2 = static_cast<size_t>(b->N[index]) * b->N[index];
+	for(int q = 0; q < count; ++q) {
+		double* blk = st.data() + static_cast<size_t>(q) * pe * 2;
+		if(ados[q] == 0 && b->precision != HELIX_HIP_FP64)
+			HX_HIP(hipMemcpy(blk, b->rho0_f64 + 2 * b->r0_off[index], pe * 16, hipMemcpyDeviceToHost));
+		from_stored(blk, 1, b->logN[index], b->herm, {}, out + static_cast<size_t>(q) * n2 * 2);
+	}
 	return 0;
 }
 

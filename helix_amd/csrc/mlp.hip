@@ -2,9 +2,10 @@
 // hidden layers) with BF16 weights, FP32 activations and FP32 accumulation, trained on the GPU by the
 // kernels below (forward, backward, Adam on FP32 master weights).  No counterpart in the reference.
 //
-// Weights are stored [out][in] (row-major, one row per output unit).  Every GEMM is the tiled kernel
-// k_gemm: C[m][n] = sum_k A(m,k) B(k,n) with A/B read through strides so transposes cost nothing;
-// BF16 operands are widened to FP32 in registers.
+// Weights are stored [out][in] (row-major, one row per output unit).  Every GEMM of the forward and backward
+// passes runs on the gfx950 matrix cores (k_gemm_mfma: v_mfma_f32_16x16x16_bf16, FP32 accumulation):
+// C[m][n] = sum_k bf16(A(m,k)) bf16(B(k,n)), A and B read through strides so transposes cost nothing, FP32
+// operands (activations, back-propagated errors) rounded to BF16 (nearest even) as they are staged in LDS.
 #include "helix_mlp.h"
 
 #include <hip/hip_bfloat16.h>
@@ -38,37 +39,99 @@ int hfail(hipError_t e, const char* what)
 		if(e_ != hipSuccess) return hfail(e_, #call);           \
 	} while(0)
 
-constexpr int kT = 16;  // GEMM tile
+// ---- BF16 MFMA GEMM --------------------------------------------------------------------------------
+// One 256-thread workgroup (4 waves) per 64 x 64 tile of C; wave w owns the 32 x 32 quarter (w / 2, w % 2) as
+// 2 x 2 MFMA tiles of 16 x 16.  K advances 32 at a time: the A tile [64][32] and the B tile, stored transposed as
+// [64 n][32 k], are staged in LDS as BF16 (rows padded to 40 elements), then each wave issues two
+// v_mfma_f32_16x16x16_bf16 per 16 x 16 tile.  Operand layout of that instruction: lane l supplies A[i][k] for
+// i = l % 16, k = 4 (l / 16) .. +3 and B[k][j] for j = l % 16, the same k; it returns D[i][j] for j = l % 16,
+// i = 4 (l / 16) + v in register v.  Rows of M tiles run along grid x (any M), N tiles along grid y.
+constexpr int kGT = 64;      // C tile
+constexpr int kGK = 32;      // K per LDS stage
+constexpr int kGLd = kGK + 8;  // padded LDS row (BF16 elements)
 
-__device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
-__device__ __forceinline__ float ld(const hip_bfloat16* p, size_t i) { return static_cast<float>(p[i]); }
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-// C[m][n] (+)= sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn]  (+ bias[n]); FP32 accumulate
+__device__ __forceinline__ unsigned short bf16_bits(float v)  // round to nearest even (finite inputs)
+{
+	const unsigned u = __float_as_uint(v);
+	return static_cast<unsigned short>((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ unsigned short bf16_bits(hip_bfloat16 v)
+{
+	unsigned short b;
+	__builtin_memcpy(&b, &v, sizeof(b));
+	return b;
+}
+
 template <typename TA, typename TB>
-__global__ __launch_bounds__(kT * kT) void k_gemm(int M, int N, int K, const TA* __restrict__ A, long sam, long sak,
+__global__ __launch_bounds__(256) void k_gemm_mfma(int M, int N, int K, const TA* __restrict__ A, long sam, long sak,
 	const TB* __restrict__ B, long sbk, long sbn, float* __restrict__ C, long ldc, const float* __restrict__ bias,
 	int accumulate)
 {
-	__shared__ float sa[kT][kT + 1];
-	__shared__ float sb[kT][kT + 1];
-	const int tx = threadIdx.x % kT, ty = threadIdx.x / kT;
-	const int m = blockIdx.y * kT + ty, n = blockIdx.x * kT + tx;
-	float acc = 0.f;
-	for(int k0 = 0; k0 < K; k0 += kT) {
-		const int ka = k0 + tx, kb = k0 + ty;
-		sa[ty][tx] = (m < M && ka < K) ? ld(A, static_cast<size_t>(m) * sam + static_cast<size_t>(ka) * sak) : 0.f;
-		const int nb = blockIdx.x * kT + tx;
-		sb[ty][tx] = (kb < K && nb < N) ? ld(B, static_cast<size_t>(kb) * sbk + static_cast<size_t>(nb) * sbn) : 0.f;
+	__shared__ unsigned short sa[kGT * kGLd];  // [m][k]
+	__shared__ unsigned short sb[kGT * kGLd];  // [n][k]
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int m0 = blockIdx.x * kGT, n0 = blockIdx.y * kGT;
+	const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+	f32x4_t acc[2][2];
+#pragma unroll
+	for(int a = 0; a < 2; ++a)
+#pragma unroll
+		for(int b = 0; b < 2; ++b) acc[a][b] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+	// staging: 64 x 32 elements per operand, 8 per thread; the unit-stride index runs fastest over the threads
+	const bool a_k_fast = sak == 1, b_k_fast = sbk == 1;
+	for(int k0 = 0; k0 < K; k0 += kGK) {
+#pragma unroll
+		for(int r = 0; r < 8; ++r) {
+			const int e = tid + 256 * r;  // 0 .. 2047
+			const int mi = a_k_fast ? e / kGK : e % kGT, ki = a_k_fast ? e % kGK : e / kGT;
+			const int gm = m0 + mi, gk = k0 + ki;
+			unsigned short v = 0;
+			if(gm < M && gk < K) v = bf16_bits(A[static_cast<size_t>(gm) * sam + static_cast<size_t>(gk) * sak]);
+			sa[mi * kGLd + ki] = v;
+			const int ni = b_k_fast ? e / kGK : e % kGT, kj = b_k_fast ? e % kGK : e / kGT;
+			const int gn = n0 + ni, gk2 = k0 + kj;
+			unsigned short w = 0;
+			if(gn < N && gk2 < K) w = bf16_bits(B[static_cast<size_t>(gk2) * sbk + static_cast<size_t>(gn) * sbn]);
+			sb[ni * kGLd + kj] = w;
+		}
 		__syncthreads();
 #pragma unroll
-		for(int k = 0; k < kT; ++k) acc += sa[ty][k] * sb[k][tx];
+		for(int kk = 0; kk < kGK; kk += 16) {
+			const int kq = kk + 4 * (lane >> 4);
+			bf16x4_t av[2], bv[2];
+#pragma unroll
+			for(int a = 0; a < 2; ++a) {
+				const unsigned short* pa = sa + (wm + 16 * a + (lane & 15)) * kGLd + kq;
+				const unsigned short* pb = sb + (wn + 16 * a + (lane & 15)) * kGLd + kq;
+				av[a] = bf16x4_t{static_cast<short>(pa[0]), static_cast<short>(pa[1]), static_cast<short>(pa[2]),
+					static_cast<short>(pa[3])};
+				bv[a] = bf16x4_t{static_cast<short>(pb[0]), static_cast<short>(pb[1]), static_cast<short>(pb[2]),
+					static_cast<short>(pb[3])};
+			}
+#pragma unroll
+			for(int a = 0; a < 2; ++a)
+#pragma unroll
+				for(int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[a], bv[b], acc[a][b], 0, 0, 0);
+		}
 		__syncthreads();
 	}
-	if(m < M && n < N) {
-		float v = acc + (bias ? bias[n] : 0.f);
-		float* c = C + static_cast<size_t>(m) * ldc + n;
-		*c = accumulate ? *c + v : v;
-	}
+#pragma unroll
+	for(int a = 0; a < 2; ++a)
+#pragma unroll
+		for(int b = 0; b < 2; ++b)
+#pragma unroll
+			for(int v = 0; v < 4; ++v) {
+				const int m = m0 + wm + 16 * a + 4 * (lane >> 4) + v;
+				const int n = n0 + wn + 16 * b + (lane & 15);
+				if(m < M && n < N) {
+					float out = acc[a][b][v] + (bias ? bias[n] : 0.f);
+					float* c = C + static_cast<size_t>(m) * ldc + n;
+					*c = accumulate ? *c + out : out;
+				}
+			}
 }
 
 __global__ void k_tanh(float* x, size_t n)
@@ -215,8 +278,9 @@ template <typename TA, typename TB>
 void gemm(hipStream_t s, int M, int N, int K, const TA* A, long sam, long sak, const TB* B, long sbk, long sbn, float* C,
 	long ldc, const float* bias, int acc)
 {
-	dim3 grid((N + kT - 1) / kT, (M + kT - 1) / kT);
-	hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(kT * kT), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias, acc);
+	dim3 grid((M + kGT - 1) / kGT, (N + kGT - 1) / kGT);
+	hipLaunchKernelGGL((k_gemm_mfma<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, bias,
+		acc);
 }
 
 // forward on m->x (n rows) -> m->y, keeping a1, a2 for the backward pass

@@ -153,6 +153,12 @@ int helix_hip_step_noalloc(helix_hip_plan* plan, uint64_t nsteps, uint64_t recor
  * (ResultExtractor::final_reduced_density, src/library/result_extractor.cu:35-89). */
 int helix_hip_get_tier0(helix_hip_plan* plan, double* out);
 
+/* Selected ADO blocks out, out[q] = ADO ados[q] (hierarchy order, 0 <= ados[q] < H) as [N][N][2] complex128, unpacked
+ * from the stored layout (FP32 forms: tier 0 from its FP64 master copy).  Generalises the reference's extraction of
+ * block 0 (ResultExtractor::final_reduced_density, src/library/result_extractor.cu:35-89) to any ADO; gathered on the
+ * device, so only the listed blocks cross PCIe. */
+int helix_hip_get_ados(helix_hip_plan* plan, const int32_t* ados, int count, double* out);
+
 /* Device-timed steps: enqueues nsteps steps with HIP events around every stage launch on the plan's
  * stream; stage_ms[j] (j < order, may be NULL) gets the mean duration of stage j+1 launches and
  * total_ms the event time of the whole sequence. */
@@ -211,6 +217,11 @@ typedef struct helix_hip_terminator {
 	int32_t iterations;          /* fixed-point rounds for S, 1..16 */
 } helix_hip_terminator;
 int helix_hip_plan_set_terminator(helix_hip_plan* plan, const helix_hip_terminator* terminator);
+/* Learned correction of the installed terminator (no counterpart in the reference): corr [U][N][N][2] multiplies
+ * the response of every eigen-element, X~_ab = -corr[u][a][b] (Q^dagger (D + S X_prev) Q)_ab / (rate[u] - i (E_a - E_b)),
+ * so corr = 1 is the physics terminator above.  NULL restores it.  EINVAL without a terminator; installing another
+ * terminator drops the correction. */
+int helix_hip_plan_set_terminator_gain(helix_hip_plan* plan, const double* corr);
 
 /* ---- batched form: many independent problems advanced by one launch per Taylor stage ----
  * No counterpart in the reference (one hard-coded problem per process, src/helix.cpp:73-104).
@@ -229,6 +240,8 @@ int helix_hip_batch_set_initial(helix_hip_batch* batch, int index, const double*
 int helix_hip_batch_step(helix_hip_batch* batch, uint64_t nsteps);
 int helix_hip_batch_get_tier0(helix_hip_batch* batch, int index, double* out);
 int helix_hip_batch_get_state(helix_hip_batch* batch, int index, double* state);
+/* helix_hip_get_ados for problem `index` of the batch */
+int helix_hip_batch_get_ados(helix_hip_batch* batch, int index, const int32_t* ados, int count, double* out);
 int helix_hip_batch_time_steps(helix_hip_batch* batch, uint64_t nsteps, double* total_ms);
 /* total ADOs, device bytes, workgroup threads chosen, graph in use */
 int helix_hip_batch_info(const helix_hip_batch* batch, int64_t* total_ado, size_t* device_bytes, int* threads,

@@ -91,28 +91,30 @@ class OracleTerminator:
             out.append(np.linalg.solve(K, -D.reshape(-1)).reshape(N, N))
         return out
 
-    def fixed_point(self, x, iterations: int) -> list:
-        """The kernel's algorithm: X <- solve of (-i ad_H + rate) X = -(D + Xi X) in the eigenbasis of H."""
+    def fixed_point(self, x, iterations: int, corr=None) -> list:
+        """The kernel's algorithm: X <- solve of (-i ad_H + rate) X = -(D + Xi X) in the eigenbasis of H; with
+        corr [U][N][N] every eigen-element's response is multiplied by corr[u] (the learned correction)."""
         E, Q = np.linalg.eigh(self.H)
         out = []
-        for (rate, _), D in zip(self.virtual, self._D(x)):
+        for u, ((rate, _), D) in enumerate(zip(self.virtual, self._D(x))):
             den = rate - 1j * (E[:, None] - E[None, :])
+            g = (1.0 if corr is None else corr[u]) / den
             X = np.zeros_like(D)
             for _ in range(iterations):
                 R = D + self.xi_self * X
-                X = Q @ (-(Q.conj().T @ R @ Q) / den) @ Q.conj().T
+                X = Q @ (-(Q.conj().T @ R @ Q) * g) @ Q.conj().T
             out.append(X)
         return out
 
-    def missing(self, x, iterations: int | None = None) -> np.ndarray:
-        X = self.exact(x) if iterations is None else self.fixed_point(x, iterations)
+    def missing(self, x, iterations: int | None = None, corr=None) -> np.ndarray:
+        X = self.exact(x) if iterations is None else self.fixed_point(x, iterations, corr)
         out = np.zeros_like(x)
         for n, k, q in self.up:
             out[n] += -1j * self.dv[k] * X[q]
         return out
 
-    def rhs(self, x, iterations: int | None = None) -> np.ndarray:
-        return self.small.rhs(x) + self.missing(x, iterations)
+    def rhs(self, x, iterations: int | None = None, corr=None) -> np.ndarray:
+        return self.small.rhs(x) + self.missing(x, iterations, corr)
 
     def taylor(self, x, dt: float, nsteps: int, iterations: int | None = None, order: int = 4) -> np.ndarray:
         """The Taylor-m recurrence (src/liouville.cu:537-583) with the terminator in every stage."""

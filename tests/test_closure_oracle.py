@@ -70,3 +70,44 @@ def test_host_terminator_tables_match_the_oracle(CO, seed):
     x = _rand(T.H, T.N, seed)
     ref = ot.missing(x, 5)
     assert np.max(np.abs(T.missing(x) - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_response_targets_are_one_at_the_stationary_point():
+    """Pin of the learned correction's training targets: when every tier-(L+1) ADO is exactly the terminator's
+    stationary solution of the kept state (fixed point iterated to convergence), the least-squares response
+    c* = -(sum conj(R~) X~ / sum |R~|^2)(r_u - i (E_a - E_b)) is 1 for every virtual ADO and eigen-element."""
+    from helix_amd import closure as cl
+    p = cl.Params.draw(9004)
+    T = cl.Terminator(p, 2, iterations=80)
+    src_small = np.unique(T.src_ado)
+    local = {int(a): q for q, a in enumerate(src_small)}
+    src_local = np.array([local[int(a)] for a in T.src_ado])
+    seg = np.asarray(T.src_offsets[:-1], np.int64)
+    num, den = 0.0, 0.0
+    for k in range(3):
+        x = _rand(T.H, T.N, 100 + k)
+        X = T.virtual(x)
+        a, b = cl.response_terms(cl.source_weights(T)[None], T.evecs[None], T._selfw[None], x[src_small][None],
+                                 X[None], src_local, seg)
+        num, den = num + a[0], den + b[0]
+    c = -(num / den) * cl.physical_response(T)
+    assert np.max(np.abs(c - 1.0)) <= 1e-9
+    # a planted mistake (the response without the Xi self term in R) is caught
+    a, b = cl.response_terms(cl.source_weights(T)[None], T.evecs[None], 0.0 * T._selfw[None], x[src_small][None],
+                             X[None], src_local, seg)
+    assert np.max(np.abs(-(a[0] / b[0]) * cl.physical_response(T) - 1.0)) > 1e-3
+
+
+def test_correction_features_layout():
+    from helix_amd import closure as cl
+    p = cl.Params.draw(9001)
+    T = cl.Terminator(p, 2)
+    F = cl.correction_features(T)
+    N = T.N
+    assert F.shape == (T.count * N * N, cl.feature_dim(4)) and F.dtype == np.float32
+    u, a, b = 7, 3, 11
+    row = F[u * N * N + a * N + b]
+    assert np.isclose(row[0], p.lam / 0.5) and np.isclose(row[1], p.gamma / 2) and np.isclose(row[2], p.T / 2)
+    assert np.allclose(row[9:17], T.virtual_rows[u] / 3.0)
+    assert np.isclose(row[17], T.rate[u].real / 10) and np.isclose(row[18], T.rate[u].imag / 10)
+    assert np.allclose(row[19:22], [T.evals[a] / 4, T.evals[b] / 4, (T.evals[a] - T.evals[b]) / 4], atol=1e-6)
