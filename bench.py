@@ -165,59 +165,75 @@ def copy_probe(nbytes: int) -> int:
 
 
 def bench_learned(args):
-    """Learned-closure form: 4 spins (N = 16), tiers 0..2 kept plus the learned closure, complex64,
-    dt 0.01, 2000 steps, against plain FP64 at tiers 0..6 (3003 ADOs) on held-out parameter sets.
-    Training (128 parameter sets propagated by the deep plain solver, 20 epochs on the GPU) is done first
-    and not timed.  value = ADO-steps/s of the learned solver; time_to_solution_x = deep plain time /
-    learned time for the same 2000 steps."""
+    """Learned-closure form: 4 spins (N = 16), tiers 0..2 kept, the physics terminator with its learned correction
+    (the committed network helix_amd/data/closure_n16_L2.npz, trained on 4096 parameter sets by
+    helix_amd/train_closure.py), complex64, dt 0.01, 2000 steps, on held-out parameter sets (seeds 9000..9005)
+    against plain FP64 at tiers 0..6 (3003 ADOs).  value = ADO-steps/s of the learned solver; beside it the error
+    (max|d rho_S| over the trajectory) of plain truncation, the physics terminator and the learned correction,
+    and time to solution against the deep plain solver and against plain FP64 at the shallowest depth whose
+    error is at most the learned closure's."""
     import numpy as np
 
     import helix_amd as hx
     from helix_amd import closure as cl
-    steps = 2000
-    train_sets = [cl.Params.draw(7000 + i) for i in range(128)]
-    X, Y = cl.training_pairs(train_sets, keep_tier=2, deep_tier=6, steps=steps, every=100)
-    mlp = cl.MLP(X.shape[1], 256, 256, 2, seed=11)
-    losses = mlp.train(X, Y, epochs=20, batch=512, lr=1e-3, seed=3)
+    path = os.path.join(ROOT, "helix_amd", "data", "closure_n16_L2.npz")
+    if not os.path.exists(path):
+        print(json.dumps({"error": f"trained network {path} not found (python -m helix_amd.train_closure)"}))
+        return 2
+    mlp = cl.load_network(path)
+    steps, every, seeds = 2000, 20, list(range(9000, 9006))
     rows = []
-    for seed in (9000, 9001, 9002, 9003, 9004, 9005):
+    for seed in seeds:
         p = cl.Params.draw(seed)
-        deep = hx.Plan(p.model(6, "fp64"))
-        deep.set_initial()
-        t0 = time.perf_counter()
-        deep.step(steps)
-        t_deep = time.perf_counter() - t0
-        ref = deep.tier0()
-        deep.close()
-        s = cl.LearnedClosureSolver(p, mlp, keep_tier=2, precision="fp32")
-        s.enable(False)
-        plain = s.run(steps)
-        s.enable(True)
-        s.plan.set_initial()
-        t0 = time.perf_counter()
-        s.plan.step(steps)
-        t_learned = time.perf_counter() - t0
-        learned = s.plan.tier0()
-        rows.append(dict(seed=seed, err_plain_L2=float(np.max(np.abs(plain - ref))),
-                         err_learned_L2=float(np.max(np.abs(learned - ref))), t_deep_s=t_deep, t_learned_s=t_learned,
-                         n_ado_learned=s.model.H, n_ado_deep=3003))
+        row = {"seed": seed}
+        ref = None
+        for L in (6, 5, 4, 3, 2):  # plain FP64 at every depth: errors and times (tiers 0..6 is the reference)
+            plan = hx.Plan(p.model(L, "fp64"))
+            plan.set_initial()
+            plan.step(10)
+            plan.set_initial()
+            t0 = time.perf_counter()
+            tr, _ = plan.step(steps, every, record_rho=True)
+            row[f"t_plain_fp64_L{L}_s"] = time.perf_counter() - t0
+            plan.close()
+            if ref is None:
+                ref = tr
+            row[f"err_plain_fp64_L{L}"] = float(np.max(np.abs(tr - ref)))
+        s = cl.ClosureSolver(p, mlp, keep_tier=2, precision="fp32")
+        for mode in ("plain", "physics", "learned"):
+            s.use(mode)
+            s.run(10)
+            t0 = time.perf_counter()
+            _, tr = s.run(steps, every)
+            row[f"t_{mode}_s"] = time.perf_counter() - t0
+            row[f"err_{mode}"] = float(np.max(np.abs(tr - ref)))
+        row["n_ado_kept"] = s.model.H
         s.close()
-    t_l = sum(r["t_learned_s"] for r in rows)
-    t_d = sum(r["t_deep_s"] for r in rows)
+        # shallowest plain FP64 depth at least as accurate as the learned closure (6 = the reference itself)
+        row["matched_plain_L"] = min(L for L in (2, 3, 4, 5, 6) if row[f"err_plain_fp64_L{L}"] <= row["err_learned"])
+        rows.append(row)
+        print(f"# seed {seed}: plain {row['err_plain']:.3e} physics {row['err_physics']:.3e} "
+              f"learned {row['err_learned']:.3e} matched plain L={row['matched_plain_L']}", file=sys.stderr, flush=True)
+    mlp.close()
+    t_learned = sum(r["t_learned_s"] for r in rows)
+    t_deep = sum(r["t_plain_fp64_L6_s"] for r in rows)
+    t_matched = sum(r[f"t_plain_fp64_L{r['matched_plain_L']}_s"] for r in rows)
+    n_kept = rows[0]["n_ado_kept"]
+    with np.load(path, allow_pickle=False) as f:
+        train_sets = int(f["train_sets"])
     line = {
-        "metric": "ADO-steps/sec on 1 MI355X", "value": rows[0]["n_ado_learned"] * steps * len(rows) / t_l,
-        "unit": "ADO-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": t_l * 1e3 / (steps * len(rows)),
+        "metric": "ADO-steps/sec on 1 MI355X", "value": n_kept * steps * len(rows) / t_learned, "unit": "ADO-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": 10, "ms_per_step": t_learned * 1e3 / (steps * len(rows)),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded spin-glass parameter sets; 128 for training, 6 held out)",
+        "data": f"synthetic (seeded spin-glass parameter sets; network trained on {train_sets}, {len(rows)} held out)",
         "config": {"workload": "learned_closure_n16_L2", "spins": 4, "N": 16, "kept_tiers": "0..2",
-                   "reference_tiers": "0..6", "train_loss_first_last": [float(losses[0]), float(losses[-1])]},
-        "time_to_solution_x": t_d / t_l,
-        "mean_err_plain_L2": float(np.mean([r["err_plain_L2"] for r in rows])),
-        "mean_err_learned_L2": float(np.mean([r["err_learned_L2"] for r in rows])),
+                   "reference_tiers": "0..6", "network": "2 x 256 tanh, BF16 weights, FP32 accumulation (BF16 MFMA)"},
+        "mean_err": {k: float(np.mean([r[f"err_{k}"] for r in rows])) for k in ("plain", "physics", "learned")},
+        "time_to_solution_x_vs_deep_plain_fp64": t_deep / t_learned,
+        "time_to_solution_x_vs_plain_fp64_at_matched_error": t_matched / t_learned,
         "held_out": rows,
     }
     print(json.dumps(line), flush=True)
-    mlp.close()
     return 0
 
 
