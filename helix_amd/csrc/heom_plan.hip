@@ -1606,6 +1606,34 @@ struct TermArgs {
 	const double2* corr;      // [U][N][N] complex factor on each eigen-element's response, or null (factor 1)
 };
 
+// The four N x N products of each round are loops over k unrolled in groups of up to 16 terms (N is a template
+// constant), so a thread issues a group's LDS reads before the FMAs that consume them and pays the LDS latency
+// once per group rather than once per term.
+template <int N>
+__device__ __forceinline__ double2 row_times_col(const double2* __restrict__ a, int lda, const double2* __restrict__ b,
+	int ldb, bool conj_a, bool conj_b)
+{
+	constexpr int G = N <= 16 ? N : 8;  // N = 32 runs 1024 threads: 8-term groups keep it within its registers
+	double re = 0.0, im = 0.0;
+#pragma unroll 1
+	for(int k0 = 0; k0 < N; k0 += G) {
+		double2 av[G], bv[G];
+#pragma unroll
+		for(int k = 0; k < G; ++k) {
+			av[k] = a[(k0 + k) * lda];
+			bv[k] = b[(k0 + k) * ldb];
+		}
+#pragma unroll
+		for(int k = 0; k < G; ++k) {
+			const double ar = av[k].x, ai = conj_a ? -av[k].y : av[k].y;
+			const double br = bv[k].x, bi = conj_b ? -bv[k].y : bv[k].y;
+			re += ar * br - ai * bi;
+			im += ar * bi + ai * br;
+		}
+	}
+	return make_double2(re, im);
+}
+
 template <typename T, int LOGN>
 __global__ __launch_bounds__(1 << (2 * LOGN)) void k_terminator(const TermArgs t)
 {
@@ -1637,57 +1665,40 @@ __global__ __launch_bounds__(1 << (2 * LOGN)) void k_terminator(const TermArgs t
 	}
 	const double2 rate = t.rate[u];
 	const double2 sw = t.selfx[i ^ j];
+	double2 corr = make_double2(1.0, 0.0);
+	if(t.corr) corr = t.corr[static_cast<size_t>(u) * E + e];
 	__syncthreads();  // s_q, s_ev
+	// response of eigen-element (a, b) = (i, j): -corr / (rate - i (E_a - E_b)), the same every round
+	double gr, gi;
+	{
+		const double den_r = rate.x, den_i = rate.y - (s_ev[i] - s_ev[j]);
+		const double inv = 1.0 / (den_r * den_r + den_i * den_i);
+		gr = -(corr.x * den_r + corr.y * den_i) * inv;
+		gi = -(corr.y * den_r - corr.x * den_i) * inv;
+	}
 	double xr = 0.0, xi = 0.0;
 	for(int it = 0; it < t.iterations; ++it) {
 		// R = D + S X_prev, then R Q
 		s_a[e] = make_double2(dr + sw.x * xr - sw.y * xi, di + sw.x * xi + sw.y * xr);
 		__syncthreads();
-		double ar = 0.0, ai = 0.0;
-		for(int k = 0; k < N; ++k) {
-			const double2 r = s_a[i * N + k], qv = s_q[k * N + j];
-			ar += r.x * qv.x - r.y * qv.y;
-			ai += r.x * qv.y + r.y * qv.x;
-		}
+		const double2 rq = row_times_col<N>(s_a + i * N, 1, s_q + j, N, false, false);
 		__syncthreads();
-		s_a[e] = make_double2(ar, ai);
+		s_a[e] = rq;
 		__syncthreads();
-		// R~ = Q^dagger (R Q), element (a, b) = (i, j); X~ = -R~ / (rate - i (E_a - E_b))
-		double tr = 0.0, ti = 0.0;
-		for(int k = 0; k < N; ++k) {
-			const double2 qv = s_q[k * N + i], av = s_a[k * N + j];
-			tr += qv.x * av.x + qv.y * av.y;
-			ti += qv.x * av.y - qv.y * av.x;
-		}
-		const double den_r = rate.x, den_i = rate.y - (s_ev[i] - s_ev[j]);
-		const double inv = 1.0 / (den_r * den_r + den_i * den_i);
-		double yr = -(tr * den_r + ti * den_i) * inv, yi = -(ti * den_r - tr * den_i) * inv;
-		if(t.corr) {  // learned correction of the response of eigen-element (a, b) = (i, j)
-			const double2 c = t.corr[static_cast<size_t>(u) * E + e];
-			const double zr = yr * c.x - yi * c.y;
-			yi = yr * c.y + yi * c.x;
-			yr = zr;
-		}
+		// R~ = Q^dagger (R Q); X~ = response * R~
+		const double2 tt = row_times_col<N>(s_q + i, N, s_a + j, N, true, false);
+		const double yr = gr * tt.x - gi * tt.y, yi = gr * tt.y + gi * tt.x;
 		__syncthreads();
 		// X = Q X~ Q^dagger
 		s_a[e] = make_double2(yr, yi);
 		__syncthreads();
-		double br = 0.0, bi = 0.0;
-		for(int k = 0; k < N; ++k) {
-			const double2 y = s_a[i * N + k], qv = s_q[j * N + k];
-			br += y.x * qv.x + y.y * qv.y;
-			bi += y.y * qv.x - y.x * qv.y;
-		}
+		const double2 yq = row_times_col<N>(s_a + i * N, 1, s_q + j * N, 1, false, true);
 		__syncthreads();
-		s_a[e] = make_double2(br, bi);
+		s_a[e] = yq;
 		__syncthreads();
-		xr = 0.0;
-		xi = 0.0;
-		for(int k = 0; k < N; ++k) {
-			const double2 qv = s_q[i * N + k], b = s_a[k * N + j];
-			xr += qv.x * b.x - qv.y * b.y;
-			xi += qv.x * b.y + qv.y * b.x;
-		}
+		const double2 xx = row_times_col<N>(s_q + i * N, 1, s_a + j, N, false, false);
+		xr = xx.x;
+		xi = xx.y;
 		__syncthreads();
 	}
 	cur[static_cast<size_t>(t.n_real + u) * E + e] = V2{static_cast<T>(xr), static_cast<T>(xi)};
