@@ -139,12 +139,37 @@ def measure_forms(args, fp64_value):
                        "steps_compared": 10 + args.form_steps}
     v64, info, r64 = _time_batch("fp64", 512, args.batch_steps)
     v32, _, r32 = _time_batch("fp32", 512, args.batch_steps)
+    seq = _time_one_at_a_time(512, 20)
     forms["batched_512"] = {"n_ado": info["total_ado"], "fp64": v64, "complex64": v32,
+                            "plain_fp64_one_plan_at_a_time": seq,
+                            "complex64_x_plain_fp64_same_problems": v32 / seq, "fp64_x_plain_fp64_same_problems": v64 / seq,
                             "complex64_x_batched_fp64": v32 / v64, "complex64_x_plain_fp64_n64": v32 / fp64_value,
                             "complex64_max_abs_drho_S": max(_max_abs(a, b) for a, b in zip(r32, r64)),
                             "steps_compared": 2 + args.batch_steps}
+    summary = learned_summary()
+    if summary is not None:
+        forms["learned_closure_n16_L2"] = summary
     print(f"# forms: {json.dumps(forms)}", file=sys.stderr, flush=True)
     return forms
+
+
+def _time_one_at_a_time(count, steps):
+    """ADO-steps/s of the plain FP64 form on the batched workload's problems, one plan at a time (each plan
+    created, warmed up, then `steps` steps timed; creation untimed)."""
+    import helix_amd as hx
+    t_all, ado_steps = 0.0, 0
+    for i in range(count):
+        m = hx.Model.spin_glass_random(1000 + i, dt=0.01, precision="fp64")
+        p = hx.Plan(m)
+        p.set_initial()
+        p.step(2)
+        t0 = time.perf_counter()
+        p.step(steps)
+        t_all += time.perf_counter() - t0
+        ado_steps += m.H * steps
+        p.close()
+        m.close()
+    return ado_steps / t_all
 
 
 def copy_probe(nbytes: int) -> int:
@@ -164,30 +189,20 @@ def copy_probe(nbytes: int) -> int:
     return nbytes
 
 
-def bench_learned(args):
-    """Learned-closure form: 4 spins (N = 16), tiers 0..2 kept, the physics terminator with its learned correction
-    (the committed network helix_amd/data/closure_n16_L2.npz, trained on 4096 parameter sets by
-    helix_amd/train_closure.py), complex64, dt 0.01, 2000 steps, on held-out parameter sets (seeds 9000..9005)
-    against plain FP64 at tiers 0..6 (3003 ADOs).  value = ADO-steps/s of the learned solver; beside it the error
-    (max|d rho_S| over the trajectory) of plain truncation, the physics terminator and the learned correction,
-    and time to solution against the deep plain solver and against plain FP64 at the shallowest depth whose
-    error is at most the learned closure's."""
+def learned_rows(mlp, seeds, steps=2000, every=20, depths=(6, 5, 4, 3, 2)):
+    """Per held-out parameter set: max|d rho_S| over the trajectory against plain FP64 at tiers 0..6 and wall time of
+    plain FP64 at every depth and of the tier-2 solver with plain truncation, the physics terminator and the learned
+    correction (complex64)."""
     import numpy as np
 
     import helix_amd as hx
     from helix_amd import closure as cl
-    path = os.path.join(ROOT, "helix_amd", "data", "closure_n16_L2.npz")
-    if not os.path.exists(path):
-        print(json.dumps({"error": f"trained network {path} not found (python -m helix_amd.train_closure)"}))
-        return 2
-    mlp = cl.load_network(path)
-    steps, every, seeds = 2000, 20, list(range(9000, 9006))
     rows = []
     for seed in seeds:
         p = cl.Params.draw(seed)
         row = {"seed": seed}
         ref = None
-        for L in (6, 5, 4, 3, 2):  # plain FP64 at every depth: errors and times (tiers 0..6 is the reference)
+        for L in depths:  # tiers 0..6 first: the reference
             plan = hx.Plan(p.model(L, "fp64"))
             plan.set_initial()
             plan.step(10)
@@ -210,9 +225,54 @@ def bench_learned(args):
         row["n_ado_kept"] = s.model.H
         s.close()
         # shallowest plain FP64 depth at least as accurate as the learned closure (6 = the reference itself)
-        row["matched_plain_L"] = min(L for L in (2, 3, 4, 5, 6) if row[f"err_plain_fp64_L{L}"] <= row["err_learned"])
+        row["matched_plain_L"] = min(L for L in sorted(depths) if row[f"err_plain_fp64_L{L}"] <= row["err_learned"])
         rows.append(row)
-        print(f"# seed {seed}: plain {row['err_plain']:.3e} physics {row['err_physics']:.3e} "
+    return rows
+
+
+NETWORK = os.path.join(ROOT, "helix_amd", "data", "closure_n16_L2.npz")
+
+
+def learned_summary(seeds=(9000, 9001, 9002, 9003, 9004, 9005)):
+    """Mean errors and time to solution of the learned closure over the held-out sets (None without the network)."""
+    import numpy as np
+
+    from helix_amd import closure as cl
+    if not os.path.exists(NETWORK):
+        return None
+    mlp = cl.load_network(NETWORK)
+    rows = learned_rows(mlp, seeds)
+    mlp.close()
+    t_l = sum(r["t_learned_s"] for r in rows)
+    return {"kept_tiers": "0..2", "reference": "plain FP64 tiers 0..6", "held_out_seeds": list(seeds),
+            "mean_err": {k: float(np.mean([r[f"err_{k}"] for r in rows])) for k in ("plain", "physics", "learned")},
+            "learned_better_than_physics": int(sum(r["err_learned"] < r["err_physics"] for r in rows)),
+            "matched_plain_L": [r["matched_plain_L"] for r in rows],
+            "time_to_solution_x_vs_deep_plain_fp64": sum(r["t_plain_fp64_L6_s"] for r in rows) / t_l,
+            "time_to_solution_x_vs_plain_fp64_at_matched_error":
+                sum(r[f"t_plain_fp64_L{r['matched_plain_L']}_s"] for r in rows) / t_l}
+
+
+def bench_learned(args):
+    """Learned-closure form: 4 spins (N = 16), tiers 0..2 kept, the physics terminator with its learned correction
+    (the committed network helix_amd/data/closure_n16_L2.npz, trained on 4096 parameter sets by
+    helix_amd/train_closure.py), complex64, dt 0.01, 2000 steps, on held-out parameter sets (seeds 9000..9005)
+    against plain FP64 at tiers 0..6 (3003 ADOs).  value = ADO-steps/s of the learned solver; beside it the error
+    (max|d rho_S| over the trajectory) of plain truncation, the physics terminator and the learned correction,
+    and time to solution against the deep plain solver and against plain FP64 at the shallowest depth whose
+    error is at most the learned closure's."""
+    import numpy as np
+
+    from helix_amd import closure as cl
+    path = NETWORK
+    if not os.path.exists(path):
+        print(json.dumps({"error": f"trained network {path} not found (python -m helix_amd.train_closure)"}))
+        return 2
+    mlp = cl.load_network(path)
+    steps, seeds = 2000, list(range(9000, 9006))
+    rows = learned_rows(mlp, seeds, steps)
+    for row in rows:
+        print(f"# seed {row['seed']}: plain {row['err_plain']:.3e} physics {row['err_physics']:.3e} "
               f"learned {row['err_learned']:.3e} matched plain L={row['matched_plain_L']}", file=sys.stderr, flush=True)
     mlp.close()
     t_learned = sum(r["t_learned_s"] for r in rows)
