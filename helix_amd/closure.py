@@ -436,10 +436,11 @@ class ClosureSolver:
     """A hierarchy kept to tiers 0..keep_tier, advanced with plain truncation ("plain"), the physics terminator
     ("physics") or the learned correction of it ("learned").  calibrate() runs a window against the plain FP64
     solver at a deep truncation and falls back to plain truncation when the closure's max|d rho_S| exceeds the
-    tolerance."""
+    tolerance.  One fixed-point round for the Xi self term by default: on the held-out sets of the trained network
+    1, 2, 3 and 5 rounds give the same max|d rho_S| to three digits, and each round costs four N x N products."""
 
     def __init__(self, params: Params, mlp: "MLP | None" = None, keep_tier: int = 2, precision: str = "fp32",
-                 dt: float = 0.01, iterations: int = 5):
+                 dt: float = 0.01, iterations: int = 1):
         self.params, self.keep_tier, self.dt = params, keep_tier, dt
         self.model = params.model(keep_tier, precision, dt)
         self.plan = Plan(self.model, layout="full")

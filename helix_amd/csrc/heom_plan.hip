@@ -2237,6 +2237,14 @@ int capture_chunk(helix_hip_plan* p)
 		drop_chunk_graph(p);
 		return hip_fail(e, "hipGraphInstantiate");
 	}
+	// upload the executable graph now, so that its first replay (possibly inside a caller's timed region) does not
+	// pay for it
+	e = hipGraphUpload(p->chunk_exec, p->stream);
+	if(e == hipSuccess) e = hipStreamSynchronize(p->stream);
+	if(e != hipSuccess) {
+		drop_chunk_graph(p);
+		return hip_fail(e, "hipGraphUpload");
+	}
 	return 0;
 }
 
@@ -3706,6 +3714,9 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 		if(e != hipSuccess) return bail(hip_fail(e, "hipStreamEndCapture"));
 		e = hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0);
 		if(e != hipSuccess) return bail(hip_fail(e, "hipGraphInstantiate"));
+		e = hipGraphUpload(b->graph_exec, b->stream);
+		if(e == hipSuccess) e = hipStreamSynchronize(b->stream);
+		if(e != hipSuccess) return bail(hip_fail(e, "hipGraphUpload"));
 		b->use_graph = true;
 	}
 	*out = b;
