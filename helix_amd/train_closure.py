@@ -84,7 +84,12 @@ def main(argv=None):
     log(f"targets: {X.shape[0]} rows x {X.shape[1]} features ({t_data:.0f} s); mean square {np.mean(Y.astype(np.float64) ** 2):.3e}")
     mlp = cl.MLP(X.shape[1], 256, 256, 2, seed=11)
     t0 = time.perf_counter()
-    losses = mlp.train(X, Y, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=3)
+    losses = []
+    for e0 in range(0, args.epochs, 4):  # a few epochs per call (Adam's state carries over), logging between them
+        n = min(4, args.epochs - e0)
+        losses.extend(float(v) for v in mlp.train(X, Y, epochs=n, batch=args.batch, lr=args.lr, seed=3 + e0))
+        log(f"epoch {e0 + n}/{args.epochs}: loss {losses[-1]:.4e}")
+    losses = np.array(losses, np.float32)
     t_train = time.perf_counter() - t0
     log(f"trained {args.epochs} epochs in {t_train:.0f} s: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
     a, b = (int(v) for v in args.held_out.split("-"))
