@@ -1704,6 +1704,125 @@ __global__ __launch_bounds__(1 << (2 * LOGN)) void k_terminator(const TermArgs t
 	cur[static_cast<size_t>(t.n_real + u) * E + e] = V2{static_cast<T>(xr), static_cast<T>(xi)};
 }
 
+// N = 16: the same terminator on the FP64 matrix cores, one wave per virtual ADO.  Lane l owns the four elements
+// (i, j) = (l / 16 + 4 v, l % 16), v = 0..3, of every 16 x 16 matrix -- the output layout of v_mfma_f64_16x16x4f64,
+// whose operands are A[i][k] from lane l with i = l % 16, k = 4 kk + l / 16, and B[k][j] with j = l % 16 (kk the
+// step of 4 in K).  A complex product is 16 MFMAs (real and imaginary planes, 4 steps each); between products the
+// result passes through LDS once to reach the operand layout.  Q and Q^dagger stay in registers in both operand
+// layouts (Q^dagger's are the conjugates of Q's in the other layout).
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+
+struct CMat16 {  // one lane's four elements of a 16 x 16 complex matrix, output layout
+	f64x4_t re, im;
+};
+
+// C = A B for A, B given per lane in operand layout (a[kk], b[kk] complex, kk = 0..3)
+__device__ __forceinline__ CMat16 cmma16(const double2 (&a)[4], const double2 (&b)[4])
+{
+	f64x4_t cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+	for(int kk = 0; kk < 4; ++kk) {
+		cr = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk].x, b[kk].x, cr, 0, 0, 0);
+		cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[kk].y, b[kk].y, cr, 0, 0, 0);
+		ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk].x, b[kk].y, ci, 0, 0, 0);
+		ci = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk].y, b[kk].x, ci, 0, 0, 0);
+	}
+	return CMat16{cr, ci};
+}
+
+// writes a lane's output-layout elements to the row-major LDS matrix and reads them back in operand layout: as A
+// (row l % 16, columns 4 kk + l / 16) or as B (rows 4 kk + l / 16, column l % 16)
+__device__ __forceinline__ void relayout16(double2* __restrict__ s, const CMat16& m, int lane, bool as_a, double2 (&out)[4])
+{
+	const int r = lane >> 4, c = lane & 15;
+#pragma unroll
+	for(int v = 0; v < 4; ++v) s[(r + 4 * v) * 16 + c] = make_double2(m.re[v], m.im[v]);
+	__syncthreads();
+#pragma unroll
+	for(int kk = 0; kk < 4; ++kk) out[kk] = as_a ? s[c * 16 + 4 * kk + r] : s[(4 * kk + r) * 16 + c];
+	__syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_terminator16(const TermArgs t)
+{
+	using V2 = typename C2t<T>::type;
+	constexpr int N = 16, E = 256;
+	__shared__ double2 s_m[E];
+	const int lane = threadIdx.x;
+	const int r = lane >> 4, c = lane & 15;
+	const int u = blockIdx.x;
+	V2* __restrict__ cur = static_cast<V2*>(t.cur);
+	// Q in both operand layouts: qa[kk] = Q[c][4kk + r] (as A), qb[kk] = Q[4kk + r][c] (as B); Q^dagger as A is
+	// conj(qb), as B conj(qa)
+	double2 qa[4], qb[4], qha[4], qhb[4];
+#pragma unroll
+	for(int kk = 0; kk < 4; ++kk) {
+		qa[kk] = t.q[c * N + 4 * kk + r];
+		qb[kk] = t.q[(4 * kk + r) * N + c];
+		qha[kk] = make_double2(qb[kk].x, -qb[kk].y);
+		qhb[kk] = make_double2(qa[kk].x, -qa[kk].y);
+	}
+	// D_u, the response and the Xi self weight of the lane's four elements
+	f64x4_t dr, di, gr, gi, swr, swi;
+	const double2 rate = t.rate[u];
+#pragma unroll
+	for(int v = 0; v < 4; ++v) {
+		const int i = r + 4 * v, j = c, e = i * N + j;
+		double ar = 0.0, ai = 0.0;
+		for(int q = t.src_off[u]; q < t.src_off[u + 1]; ++q) {
+			const V2 x = cur[static_cast<size_t>(t.src_ado[q]) * E + e];
+			const int m = t.src_mask[q];
+			const double al = t.src_alpha[q];
+			const double vi = (i & m) ? al : -al, vj = (j & m) ? al : -al;
+			const double d = vi - vj, sm = vi + vj;
+			const double2 cc = t.src_comm[q], an = t.src_anti[q];
+			const double wr = cc.x * d + an.x * sm, wi = cc.y * d + an.y * sm;
+			const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
+			ar += wr * xr - wi * xi;
+			ai += wr * xi + wi * xr;
+		}
+		dr[v] = ar;
+		di[v] = ai;
+		double2 corr = make_double2(1.0, 0.0);
+		if(t.corr) corr = t.corr[static_cast<size_t>(u) * E + e];
+		const double den_r = rate.x, den_i = rate.y - (t.ev[i] - t.ev[j]);
+		const double inv = 1.0 / (den_r * den_r + den_i * den_i);
+		gr[v] = -(corr.x * den_r + corr.y * den_i) * inv;
+		gi[v] = -(corr.y * den_r - corr.x * den_i) * inv;
+		const double2 sw = t.selfx[i ^ j];
+		swr[v] = sw.x;
+		swi[v] = sw.y;
+	}
+	CMat16 x{{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+	double2 op[4];
+	for(int it = 0; it < t.iterations; ++it) {
+		CMat16 m;  // R = D + S X
+#pragma unroll
+		for(int v = 0; v < 4; ++v) {
+			m.re[v] = dr[v] + swr[v] * x.re[v] - swi[v] * x.im[v];
+			m.im[v] = di[v] + swr[v] * x.im[v] + swi[v] * x.re[v];
+		}
+		relayout16(s_m, m, lane, true, op);
+		m = cmma16(op, qb);              // R Q
+		relayout16(s_m, m, lane, false, op);
+		m = cmma16(qha, op);             // Q^dagger (R Q)
+#pragma unroll
+		for(int v = 0; v < 4; ++v) {  // X~ = response * R~
+			const double yr = gr[v] * m.re[v] - gi[v] * m.im[v], yi = gr[v] * m.im[v] + gi[v] * m.re[v];
+			m.re[v] = yr;
+			m.im[v] = yi;
+		}
+		relayout16(s_m, m, lane, true, op);
+		m = cmma16(op, qhb);             // X~ Q^dagger
+		relayout16(s_m, m, lane, false, op);
+		x = cmma16(qa, op);              // Q (X~ Q^dagger)
+	}
+#pragma unroll
+	for(int v = 0; v < 4; ++v)
+		cur[static_cast<size_t>(t.n_real + u) * E + (r + 4 * v) * N + c] = V2{static_cast<T>(x.re[v]), static_cast<T>(x.im[v])};
+}
+
 // Learned closure: feature rows for every (listed ADO a, element e = (i, j)):
 //   x = [n_b(i) for the logN spins (as +-1), n_b(j) likewise, extra[a][0 .. extra_dim)]
 __global__ __launch_bounds__(kBlock) void k_kappa_features(int count, int logN, const float* __restrict__ extra,
@@ -2166,7 +2285,7 @@ void launch_terminator(helix_hip_plan* p, const void* cur)
 	const dim3 grid(static_cast<unsigned>(p->term_count));
 	switch(p->logN) {
 	case 3: hipLaunchKernelGGL((k_terminator<T, 3>), grid, dim3(64), 0, p->stream, t); break;
-	case 4: hipLaunchKernelGGL((k_terminator<T, 4>), grid, dim3(256), 0, p->stream, t); break;
+	case 4: hipLaunchKernelGGL((k_terminator16<T>), grid, dim3(64), 0, p->stream, t); break;
 	default: hipLaunchKernelGGL((k_terminator<T, 5>), grid, dim3(1024), 0, p->stream, t); break;
 	}
 }
