@@ -160,8 +160,8 @@ def test_learned_closure_gains_on_held_out_sets(cl, network):
         print(f"seed {seed}: " + "  ".join(row))
     mean = {k: float(np.mean(v)) for k, v in errs.items()}
     print("mean max|d rho_S|: " + "  ".join(f"{k} {v:.3e}" for k, v in mean.items()))
-    assert mean["learned"] < mean["physics"] < mean["plain"]
-    assert sum(a < b for a, b in zip(errs["learned"], errs["physics"])) >= 4
+    assert mean["learned"] < 0.7 * mean["physics"] and mean["physics"] < mean["plain"]
+    assert sum(a < b for a, b in zip(errs["learned"], errs["physics"])) >= 5
 
 
 def test_closure_falls_back_to_plain_truncation(cl, network):
