@@ -269,6 +269,18 @@ def bench_learned(args):
         print(json.dumps({"error": f"trained network {path} not found (python -m helix_amd.train_closure)"}))
         return 2
     mlp = cl.load_network(path)
+    if args.closure_only:  # the learned solver alone (seed 9000, --steps steps), e.g. under a profiler
+        s = cl.ClosureSolver(cl.Params.draw(9000), mlp, keep_tier=2, precision="fp32")
+        s.use("learned")
+        s.run(10)
+        t0 = time.perf_counter()
+        s.run(args.steps)
+        sec = time.perf_counter() - t0
+        print(json.dumps({"workload": "learned_closure_n16_L2", "seed": 9000, "steps": args.steps,
+                          "ms_per_step": sec * 1e3 / args.steps, "ado_steps_per_s": s.model.H * args.steps / sec}))
+        s.close()
+        mlp.close()
+        return 0
     steps, seeds = 2000, list(range(9000, 9006))
     rows = learned_rows(mlp, seeds, steps)
     for row in rows:
@@ -374,6 +386,8 @@ def main(argv=None):
     ap.add_argument("--timed-steps", type=int, default=100, help="steps timed per stage launch with HIP events")
     ap.add_argument("--dense-hamiltonian", action="store_true",
                     help="route -i[H, rho] through the dense MFMA commutator (N = 16, 32, 64)")
+    ap.add_argument("--closure-only", action="store_true",
+                    help="--workload learned: run only the learned solver (seed 9000) for --steps steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the FP32 / split-FP32 / batched lines measured beside the primary workload")

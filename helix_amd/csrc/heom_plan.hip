@@ -984,7 +984,13 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = static_cast<size_t>(n) * PE + qe;
 		if constexpr(!LAST) {
-			st_stream<T>(out + g, tr, ti);
+			// complex64 inner stages: ordinary stores measured 3% faster than non-temporal ones on the N = 64
+			// Hermitian tile (their output is the next stage's input); complex128 keeps the streaming store (3.5% the
+			// other way)
+			if constexpr(sizeof(T) == 4)
+				out[g] = V2{tr, ti};
+			else
+				st_stream<T>(out + g, tr, ti);
 		} else {
 			const int nt = a.ntprev;
 			if(a.rho0_f64 != nullptr && n == 0) {
