@@ -141,6 +141,7 @@ def measure_forms(args, fp64_value):
     v32, _, r32 = _time_batch("fp32", 512, args.batch_steps)
     seq = _time_one_at_a_time(512, 20)
     forms["batched_512"] = {"n_ado": info["total_ado"], "fp64": v64, "complex64": v32,
+                            "launches_per_step_fp64": info["launches_per_step"],
                             "plain_fp64_one_plan_at_a_time": seq,
                             "complex64_x_plain_fp64_same_problems": v32 / seq, "fp64_x_plain_fp64_same_problems": v64 / seq,
                             "complex64_x_batched_fp64": v32 / v64, "complex64_x_plain_fp64_n64": v32 / fp64_value,
@@ -364,7 +365,8 @@ def bench_batched(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded random spin-glass instances)",
         "config": {"workload": f"batched_{len(models)}", "instances": len(models), "n_ado": info["total_ado"],
-                   "precision": args.precision, "threads": info["threads"], "graph": info["graph"]},
+                   "precision": args.precision, "threads": info["threads"], "graph": info["graph"],
+                   "launches_per_step": info["launches_per_step"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_batch_stage",
                      "basis": "14 P s bytes per ADO-step, P stored elements per ADO of the batch's "

@@ -29,7 +29,7 @@ HIP_SYMBOLS = [
     "helix_hip_batch_get_tier0", "helix_hip_batch_get_state", "helix_hip_batch_time_steps", "helix_hip_batch_info",
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
     "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc", "helix_hip_plan_set_terminator",
-    "helix_hip_plan_set_terminator_gain", "helix_hip_get_ados", "helix_hip_batch_get_ados",
+    "helix_hip_plan_set_terminator_gain", "helix_hip_get_ados", "helix_hip_batch_get_ados", "helix_hip_batch_launches",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
@@ -129,6 +129,7 @@ def hip():
         L.helix_hip_batch_destroy.restype = None
         L.helix_hip_plan_layout.argtypes = [vp, _ip]
         L.helix_hip_batch_layout.argtypes = [vp, _ip]
+        L.helix_hip_batch_launches.argtypes = [vp, _ip]
         _hip = L
     return _hip
 
@@ -445,18 +446,20 @@ class Plan:
             pass
 
 
-BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
+BATCH_GRAPH, BATCH_AUTOTUNE, BATCH_PER_SIZE, BATCH_PERSISTENT = 1, 2, 4, 8
 
 
 class Batch:
     """Many independent problems advanced together: one launch per Taylor stage for all of them."""
 
-    def __init__(self, models, device: int = 0, graph: bool = True, autotune: bool = True, layout: str = "auto"):
+    def __init__(self, models, device: int = 0, graph: bool = True, autotune: bool = True, layout: str = "auto",
+                 per_size: bool = False, persistent: bool = False):
         self.models = list(models)
         descs = [_with_layout(m, layout) for m in self.models]
         arr = (C.c_void_p * len(descs))(*[C.cast(C.pointer(d), C.c_void_p) for d in descs])
         self._b = C.c_void_p()
-        flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        flags = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0) | (BATCH_PER_SIZE if per_size else 0)
+        flags |= BATCH_PERSISTENT if persistent else 0
         _check(hip().helix_hip_batch_create(arr, len(self.models), device, flags, C.byref(self._b)),
                "helix_hip_batch_create")
 
@@ -502,7 +505,10 @@ class Batch:
     def info(self) -> dict:
         a, b, t, g = C.c_int64(), C.c_size_t(), C.c_int(), C.c_int()
         _check(hip().helix_hip_batch_info(self._b, C.byref(a), C.byref(b), C.byref(t), C.byref(g)), "batch_info")
-        return dict(total_ado=a.value, device_bytes=b.value, threads=t.value, graph=bool(g.value))
+        n = C.c_int32()
+        _check(hip().helix_hip_batch_launches(self._b, C.byref(n)), "batch_launches")
+        return dict(total_ado=a.value, device_bytes=b.value, threads=t.value, graph=bool(g.value),
+                    launches_per_step=n.value)
 
     def close(self):
         if self._b:

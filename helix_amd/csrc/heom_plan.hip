@@ -570,7 +570,8 @@ __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int&
 }
 
 template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
-__device__ __forceinline__ void tile_body(const StageArgs& a, const int group, const TileLds<T>& lds)
+__device__ __forceinline__ void tile_body(const StageArgs& a, const int group, const TileLds<T>& lds,
+	const int tid = static_cast<int>(threadIdx.x))
 {
 	using V2 = typename C2t<T>::type;
 	constexpr int N = 1 << LOGN;
@@ -601,7 +602,6 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	WT* __restrict__ s_wt = static_cast<WT*>(lds.wt);
 	int* __restrict__ s_spread = lds.spread;
 
-	const int tid = threadIdx.x;
 	const V2* __restrict__ cur = static_cast<const V2*>(a.cur);
 	const int H = a.n_ado, M = a.M, B = a.B;
 	const int slot = tid / TA;
@@ -1103,6 +1103,76 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	const int2 tk = tasks[t];
 	tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM>(table[tk.x], tk.y,
 		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread});
+}
+
+// Persistent batched stage: one launch per Taylor stage covers the tiles of every block size.  The grid is the
+// number of workgroups the GPU holds at once; workgroups are dealt to the block sizes in proportion to their stored
+// bytes (slots [wg_off[g], wg_off[g+1]) serve block size N = 2^(3+g), each a multiple of 8), and within its block
+// size workgroup w serves the queue of XCD w % 8 (dispatch slot w runs on XCD w % 8), taking that queue's next tile
+// from an atomic counter until the queue is empty.  Each workgroup runs one block size's loop only, so the four
+// specialised tile bodies are separate code paths.  The thread index reaches the tile body through an empty asm
+// statement in every iteration, so the compiler recomputes each task's per-slot element indices instead of keeping
+// them live across tasks (hoisted, they made the kernel spill 100-180 VGPRs; each body alone needs 54-60).
+// Instances are assigned whole to a queue, balanced by stored bytes, so an instance's neighbour blocks are read
+// through one L2.  Task = (instance, tile group).  The workgroups of slots 0..31 reset the next stage's counters
+// (that launch starts after this one ends), so every launch finds its own counters at zero.
+struct BatchQueues {
+	const int2* tasks;    // queue (g, x) is tasks[qoff[8 g + x] .. qoff[8 g + x + 1])
+	const int* qoff;      // [4 * 8 + 1]
+	int wg_off[5];        // workgroup slots of each block size
+};
+
+template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM, int LOGN>
+__device__ __forceinline__ void run_queue(const StageArgs* __restrict__ table, const BatchQueues& bq, int x,
+	unsigned* __restrict__ counter, int* s_task, const TileLds<T>& lds)
+{
+	constexpr int g = LOGN - kBatchMinLogN;
+	const int begin = bq.qoff[g * kXcds + x], count = bq.qoff[g * kXcds + x + 1] - begin;
+	for(;;) {
+		if(threadIdx.x == 0) *s_task = static_cast<int>(atomicAdd(counter, 1u));
+		__syncthreads();
+		const int t = *s_task;
+		if(t >= count) break;
+		const int2 tk = bq.tasks[begin + t];
+		int tid = static_cast<int>(threadIdx.x);
+		asm volatile("" : "+v"(tid));
+		tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM>(table[tk.x], tk.y, lds, tid);
+		__syncthreads();  // the next tile stages into the LDS this one read (and thread 0 rewrites s_task)
+	}
+}
+
+template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, 6, THREADS, true, HERM>(),
+	tile_waves<T, 6, THREADS, true, HERM>()))) void
+k_batch_persistent(const StageArgs* __restrict__ table, const BatchQueues bq, unsigned* __restrict__ counters,
+	unsigned* __restrict__ next_counters)
+{
+	using V2 = typename C2t<T>::type;
+	constexpr int kNb = 64 * 2 * kTileMaxModes;  // neighbour-list entries of the tile with the most ADOs (N = 8)
+	__shared__ V2 s_blk[kTile];
+	__shared__ double s_v[1];
+	__shared__ double2 s_selfx[64];
+	// tiles of several ADOs keep their neighbour lists here, tiles of one ADO their weight table
+	__shared__ double2 s_nbwt[kNb / 4];
+	__shared__ int s_cnt[64];
+	__shared__ int s_mb[kTileMaxModes];
+	__shared__ double s_ka[kTileMaxModes];
+	__shared__ double s_hd[64];
+	__shared__ int s_spread[64];
+	__shared__ int s_task;
+	static_assert(2 * kTileMaxModes * 4 * sizeof(typename PatternWeight<T, SPLIT>::type) <= sizeof(s_nbwt), "weight table");
+	const int w = blockIdx.x;
+	if(w < 4 * kXcds && threadIdx.x == 0) next_counters[w] = 0u;
+	const int x = w % kXcds;
+	const TileLds<T> lds{s_blk, s_v, s_selfx, reinterpret_cast<int*>(s_nbwt), s_cnt, s_mb, s_ka, s_hd, s_nbwt, s_spread};
+	if(w >= bq.wg_off[3])
+		run_queue<T, SPLIT, LAST, THREADS, HERM, 6>(table, bq, x, counters + 3 * kXcds + x, &s_task, lds);
+	else if(w >= bq.wg_off[2])
+		run_queue<T, SPLIT, LAST, THREADS, HERM, 5>(table, bq, x, counters + 2 * kXcds + x, &s_task, lds);
+	else if(w >= bq.wg_off[1])
+		run_queue<T, SPLIT, LAST, THREADS, HERM, 4>(table, bq, x, counters + 1 * kXcds + x, &s_task, lds);
+	else
+		run_queue<T, SPLIT, LAST, THREADS, HERM, 3>(table, bq, x, counters + x, &s_task, lds);
 }
 
 // Dense-Hamiltonian stage for N in {16, 32, 64}: one ADO per 256-thread workgroup.
@@ -3523,6 +3593,14 @@ struct helix_hip_batch {
 	int task_cnt[4] = {};
 	int group_threads[4] = {};  // workgroup shape of each group (512 or 1024)
 	int threads = 0;            // shape of the group with the most tasks (reported by helix_hip_batch_info)
+	// persistent form: one launch per stage from per-(block size, XCD) task queues (k_batch_persistent)
+	bool persistent = false;
+	int pthreads = 0;           // its workgroup shape
+	int pgrid[2] = {};          // resident workgroups at 512 / 1024 threads (a multiple of 8)
+	int2* ptasks = nullptr;     // queue of block size g and XCD x: ptasks[pqoff[8 g + x] .. pqoff[8 g + x + 1])
+	int* pqoff = nullptr;       // [33]
+	double pcost[4] = {};       // stored bytes of each block size's instances (workgroups dealt in proportion)
+	unsigned* pcounters = nullptr;  // [order][4][8]
 	bool use_graph = false;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t graph_exec = nullptr;
@@ -3580,13 +3658,99 @@ void batch_launch_h(helix_hip_batch* b, int j)
 	batch_launch_n<T, SPLIT, HERM, 3>(b, j);
 }
 
+template <typename T, bool SPLIT, int THREADS, bool HERM>
+void batch_launch_persistent(helix_hip_batch* b, int j)
+{
+	const StageArgs* tab = b->table + static_cast<size_t>(j - 1) * b->Q;
+	unsigned* cnt = b->pcounters + static_cast<size_t>(j - 1) * 4 * kXcds;
+	unsigned* next = b->pcounters + static_cast<size_t>(j % b->order) * 4 * kXcds;
+	const int total = b->pgrid[THREADS == 1024 ? 1 : 0];
+	// workgroups dealt to the block sizes in proportion to their stored bytes, at least 8 for a size with work
+	BatchQueues bq{b->ptasks, b->pqoff, {}};
+	double all = 0.0;
+	int present = 0;
+	for(int g = 0; g < 4; ++g) {
+		all += b->pcost[g];
+		present += b->pcost[g] > 0.0 ? 1 : 0;
+	}
+	const int spare = std::max(0, total / kXcds - present);
+	bq.wg_off[0] = 0;
+	for(int g = 0; g < 4; ++g) {
+		int units = 0;
+		if(b->pcost[g] > 0.0) units = 1 + static_cast<int>(spare * (b->pcost[g] / all));
+		bq.wg_off[g + 1] = bq.wg_off[g] + units * kXcds;
+	}
+	const dim3 grid(static_cast<unsigned>(bq.wg_off[4]));
+	if(j == b->order)
+		hipLaunchKernelGGL((k_batch_persistent<T, SPLIT, true, THREADS, HERM>), grid, dim3(THREADS), 0, b->stream, tab, bq,
+			cnt, next);
+	else
+		hipLaunchKernelGGL((k_batch_persistent<T, SPLIT, false, THREADS, HERM>), grid, dim3(THREADS), 0, b->stream, tab, bq,
+			cnt, next);
+}
+
+template <typename T, bool SPLIT, bool HERM>
+void batch_launch_ph(helix_hip_batch* b, int j)
+{
+	if(b->pthreads == 1024)
+		batch_launch_persistent<T, SPLIT, 1024, HERM>(b, j);
+	else
+		batch_launch_persistent<T, SPLIT, 512, HERM>(b, j);
+}
+
 template <typename T, bool SPLIT>
 void batch_launch_p(helix_hip_batch* b, int j)
 {
+	if(b->persistent) {
+		if(b->herm)
+			batch_launch_ph<T, SPLIT, true>(b, j);
+		else
+			batch_launch_ph<T, SPLIT, false>(b, j);
+		return;
+	}
 	if(b->herm)
 		batch_launch_h<T, SPLIT, true>(b, j);
 	else
 		batch_launch_h<T, SPLIT, false>(b, j);
+}
+
+// resident workgroups of the persistent stage kernel at `threads` (the inner-stage instance; the last-stage one has
+// the same resources), a multiple of the 8 XCDs
+template <typename T, bool SPLIT, bool HERM>
+int persistent_grid(int device, int threads, int* grid)
+{
+	int cus = 0, per_cu = 0;
+	HX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+	if(threads == 1024)
+		HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<T, SPLIT, false, 1024, HERM>, 1024, 0));
+	else
+		HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_persistent<T, SPLIT, false, 512, HERM>, 512, 0));
+	*grid = std::max(kXcds, ((std::max(per_cu, 1) * cus) / kXcds) * kXcds);
+	return 0;
+}
+
+int batch_persistent_grids(helix_hip_batch* b)
+{
+	for(int k = 0; k < 2; ++k) {
+		const int th = k ? 1024 : 512;
+		int rc = 0;
+		switch(b->precision) {
+		case HELIX_HIP_FP64:
+			rc = b->herm ? persistent_grid<double, false, true>(b->device, th, &b->pgrid[k])
+			             : persistent_grid<double, false, false>(b->device, th, &b->pgrid[k]);
+			break;
+		case HELIX_HIP_FP32:
+			rc = b->herm ? persistent_grid<float, false, true>(b->device, th, &b->pgrid[k])
+			             : persistent_grid<float, false, false>(b->device, th, &b->pgrid[k]);
+			break;
+		default:
+			rc = b->herm ? persistent_grid<float, true, true>(b->device, th, &b->pgrid[k])
+			             : persistent_grid<float, true, false>(b->device, th, &b->pgrid[k]);
+			break;
+		}
+		if(rc) return rc;
+	}
+	return 0;
 }
 
 void batch_enqueue_step(helix_hip_batch* b)
@@ -3799,6 +3963,50 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 	if((rc = batch_alloc(b, &dk, tasks.size() * sizeof(int2)))) return bail(rc);
 	HX_HIP(hipMemcpy(dk, tasks.data(), tasks.size() * sizeof(int2), hipMemcpyHostToDevice));
 	b->tasks = static_cast<int2*>(dk);
+	{
+		// persistent form: per block size, instances to its 8 XCD queues, largest first to the least loaded queue
+		// (by stored bytes)
+		std::vector<int2> pt;
+		int qoff[4 * kXcds + 1];
+		for(int g = 0; g < 4; ++g) {
+			std::vector<int> qs;
+			for(int q = 0; q < count; ++q)
+				if(b->logN[q] == kBatchMinLogN + g) qs.push_back(q);
+			std::stable_sort(qs.begin(), qs.end(), [&](int u, int v) {
+				return static_cast<double>(b->H[u]) * b->pe[u] > static_cast<double>(b->H[v]) * b->pe[v];
+			});
+			std::vector<std::vector<int>> queue(kXcds);
+			double load[kXcds] = {};
+			for(int q : qs) {
+				const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
+				queue[x].push_back(q);
+				load[x] += static_cast<double>(b->H[q]) * b->pe[q];
+				b->pcost[g] += static_cast<double>(b->H[q]) * b->pe[q];
+			}
+			for(int x = 0; x < kXcds; ++x) {
+				qoff[g * kXcds + x] = static_cast<int>(pt.size());
+				for(int q : queue[x]) {
+					const int E = 1 << (2 * b->logN[q]);
+					const int A = (E >= kTile) ? 1 : kTile / E;
+					const int ngroups = (b->H[q] + A - 1) / A;
+					for(int t = 0; t < ngroups; ++t) pt.push_back(make_int2(q, t));
+				}
+			}
+		}
+		qoff[4 * kXcds] = static_cast<int>(pt.size());
+		void* d = nullptr;
+		if((rc = batch_alloc(b, &d, pt.size() * sizeof(int2)))) return bail(rc);
+		HX_HIP(hipMemcpy(d, pt.data(), pt.size() * sizeof(int2), hipMemcpyHostToDevice));
+		b->ptasks = static_cast<int2*>(d);
+		if((rc = batch_alloc(b, &d, sizeof(qoff)))) return bail(rc);
+		HX_HIP(hipMemcpy(d, qoff, sizeof(qoff), hipMemcpyHostToDevice));
+		b->pqoff = static_cast<int*>(d);
+		const size_t cbytes = static_cast<size_t>(b->order) * 4 * kXcds * sizeof(unsigned);
+		if((rc = batch_alloc(b, &d, cbytes))) return bail(rc);
+		HX_HIP(hipMemset(d, 0, cbytes));
+		b->pcounters = static_cast<unsigned*>(d);
+		if((rc = batch_persistent_grids(b))) return bail(rc);
+	}
 	HX_HIP(hipMemsetAsync(b->rho, 0, b->total_elems * b->esize, b->stream));
 	for(int q = 1; q < b->order; ++q) HX_HIP(hipMemsetAsync(b->t[q], 0, b->total_elems * b->esize, b->stream));
 	if(b->rho0_f64) HX_HIP(hipMemsetAsync(b->rho0_f64, 0, r0 * 16, b->stream));
@@ -3832,6 +4040,31 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 			if(b->task_cnt[g] > b->task_cnt[most]) most = g;
 		b->threads = b->group_threads[most];
 	}
+	// one launch per block size and stage by default (measured faster on the 512-instance workload: 13.0 M against
+	// 11.4 M ADO-steps/s FP64, 19.9 M against 14.9 M complex64); HELIX_HIP_BATCH_PERSISTENT asks for one persistent
+	// launch per stage, and the autotuner times the persistent launch at both shapes against the per-size launches
+	// tuned above and keeps the fastest of the three (HELIX_HIP_BATCH_PER_SIZE rules the persistent launch out)
+	b->persistent = (flags & HELIX_HIP_BATCH_PERSISTENT) && !(flags & HELIX_HIP_BATCH_PER_SIZE);
+	b->pthreads = (b->precision == HELIX_HIP_FP64) ? 1024 : 512;
+	if((flags & HELIX_HIP_BATCH_AUTOTUNE) && !(flags & HELIX_HIP_BATCH_PER_SIZE)) {
+		double best = 1e300;
+		bool pick_p = true;
+		int pick_t = b->pthreads;
+		for(int mode = 0; mode < 3; ++mode) {
+			b->persistent = mode > 0;
+			b->pthreads = mode == 2 ? 1024 : 512;
+			double ms = 0.0;
+			if((rc = batch_time(b, 1, &ms)) || (rc = batch_time(b, 3, &ms))) return bail(rc);
+			if(ms < best) {
+				best = ms;
+				pick_p = b->persistent;
+				pick_t = b->pthreads;
+			}
+		}
+		b->persistent = pick_p;
+		b->pthreads = pick_t;
+	}
+	if(b->persistent) b->threads = b->pthreads;
 	if(flags & HELIX_HIP_BATCH_GRAPH) {
 		HX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
 		batch_enqueue_step(b);
@@ -3951,6 +4184,18 @@ int helix_hip_batch_info(const helix_hip_batch* b, int64_t* total_ado, size_t* d
 	if(device_bytes) *device_bytes = b->device_bytes;
 	if(threads) *threads = b->threads;
 	if(graph) *graph = b->use_graph ? 1 : 0;
+	return 0;
+}
+
+int helix_hip_batch_launches(const helix_hip_batch* b, int32_t* launches_per_step)
+{
+	if(!b || !launches_per_step) return fail(HELIX_HIP_EINVAL, "null argument");
+	int per_stage = 1;
+	if(!b->persistent) {
+		per_stage = 0;
+		for(int g = 0; g < 4; ++g) per_stage += b->task_cnt[g] > 0 ? 1 : 0;
+	}
+	*launches_per_step = per_stage * b->order;
 	return 0;
 }
 

@@ -230,6 +230,8 @@ int helix_hip_plan_set_terminator_gain(helix_hip_plan* plan, const double* corr)
  * Hermitian layout when all of them allow it and none asks for HELIX_HIP_LAYOUT_FULL.  Flags: */
 #define HELIX_HIP_BATCH_GRAPH 1u     /* capture the m stage launches of a step in a hipGraph */
 #define HELIX_HIP_BATCH_AUTOTUNE 2u  /* time the workgroup shapes at creation and keep the fastest */
+#define HELIX_HIP_BATCH_PER_SIZE 4u  /* one launch per block size and stage only (the autotuner does not try the persistent launch) */
+#define HELIX_HIP_BATCH_PERSISTENT 8u  /* one persistent launch per stage from per-XCD task queues */
 
 typedef struct helix_hip_batch helix_hip_batch;
 
@@ -247,6 +249,10 @@ int helix_hip_batch_time_steps(helix_hip_batch* batch, uint64_t nsteps, double* 
 int helix_hip_batch_info(const helix_hip_batch* batch, int64_t* total_ado, size_t* device_bytes, int* threads,
 	int* graph);
 int helix_hip_batch_layout(const helix_hip_batch* batch, int32_t* layout);
+/* Kernel launches per step: the order m when every stage is one persistent launch (HELIX_HIP_BATCH_PERSISTENT, or
+ * the autotuner's pick: workgroups dealt to the block sizes in proportion to their work take tiles from per-XCD task
+ * queues), else m times the number of block sizes present (the default). */
+int helix_hip_batch_launches(const helix_hip_batch* batch, int32_t* launches_per_step);
 void helix_hip_batch_destroy(helix_hip_batch* batch);
 
 /* Message of the calling thread's last error ("" when none). */

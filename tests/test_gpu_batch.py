@@ -82,3 +82,27 @@ def test_batch_complex64_form_error(hx):
 def test_batch_rejects_bit_flip_couplings(hx):
     with pytest.raises(hx.HelixError):
         hx.Batch([hx.Model.legacy(N=32)])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_persistent_launch_equals_per_size_launches(hx, precision):
+    """One persistent launch per stage (per-XCD task queues) and one launch per block size run the same tile bodies:
+    bit-identical states, and equal to single plans; the persistent form makes one launch per stage."""
+    seeds = list(range(1000, 1024))
+    runs = {}
+    for per_size in (False, True):
+        b = hx.Batch(_models(hx, seeds, precision), graph=True, autotune=False, per_size=per_size, persistent=not per_size)
+        b.set_initial()
+        b.step(12)
+        info = b.info()
+        assert info["launches_per_step"] == (4 * len({m.N for m in b.models}) if per_size else 4), info
+        runs[per_size] = [b.state(q) for q in range(len(seeds))]
+        b.close()
+    for q in range(len(seeds)):
+        assert np.array_equal(runs[False][q], runs[True][q]), f"instance {q}"
+    for q in (0, 7, 23):
+        plan = hx.Plan(hx.Model.spin_glass_random(seeds[q], dt=0.01, precision=precision))
+        plan.set_initial()
+        plan.step(12)
+        assert np.array_equal(runs[False][q], plan.get_state())
+        plan.close()
