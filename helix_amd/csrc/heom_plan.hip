@@ -745,6 +745,62 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		}
 	};
 	if constexpr(TABLE) load_batch();
+	// -i[H, rho_n] accumulates in (hr, hi); row bands add their far row flips' partners first (below)
+	T hr[RT], hi[RT];
+#pragma unroll
+	for(int r = 0; r < RT; ++r) {
+		hr[r] = T(0);
+		hi[r] = T(0);
+	}
+	if constexpr(HERM && TPA > 1) {
+		// Row flips by a mask m >= R leave the band: the partner of stored element (i, j) is (i ^ m, j).  Where
+		// (j - (i ^ m)) mod N < N/2 it is stored in row i ^ m, and the owners' own lanes (consecutive d) read it
+		// contiguously.  Otherwise it is the conjugate of an element stored in row j; for those, lanes walk the band
+		// rows at a fixed column j (their partners i ^ m are consecutive offsets of row j), so each load is a whole
+		// run of row j, and park the values in the LDS tile (not staged yet) at their owner's slot: owner (i, j),
+		// d = (j - i) mod N, sits at (i - r0) N/2 + d, a tail owner (d = N/2, i < N/2) at R N/2 + (i - r0).  Each
+		// owner then adds h_f x (partner); the column-flip part of these flips follows from the staged tile.
+		const int r0 = ebase >> LOGN;
+		const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;
+		for(int f = 0; f < a.hflips; ++f) {
+			const int m = a.hmask[f];
+			if((m << LOGN) < kTile) continue;  // flip within the band: from the staged tile
+			if(live) {
+				for(int t = tid; t < kTile; t += THREADS) {
+					const int c = t / R, il = t - (t / R) * R;
+					const int i = r0 + il;
+					const int d = (c - i) & (N - 1);
+					int ql;
+					if(d < N / 2)
+						ql = (il << (LOGN - 1)) | d;
+					else if(d == N / 2 && i < N / 2)
+						ql = R * (N / 2) + il;
+					else
+						continue;  // (i, c) is not stored in this band
+					const int pa = i ^ m;
+					if(((c - pa) & (N - 1)) < N / 2) continue;  // stored in row i ^ m: its owner reads it
+					bool cj;
+					V2 v = ownh[herm_index(pa, c, LOGN, cj)];
+					if(cj) v.y = -v.y;
+					s_blk[ql] = v;
+				}
+			}
+			__syncthreads();
+			if(live) {
+				const Weight<T, SPLIT> w = Weight<T, SPLIT>::make(a.hcoef[f]);
+#pragma unroll
+				for(int r = 0; r < RT; ++r) {
+					if(!HX_SLOT_LIVE(r)) continue;
+					int q, i, j;
+					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+					const int pa = i ^ m, dp = (j - pa) & (N - 1);
+					const V2 x1 = (dp < N / 2) ? ownh[(pa << (LOGN - 1)) | dp] : s_blk[(r < EPT) ? lt + TA * r : R * (N / 2) + lt];
+					wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x, x1.y);
+				}
+			}
+			__syncthreads();  // the next far flip (or the staging below) overwrites the parked partners
+		}
+	}
 	if constexpr(HERM && TPA > 1) {
 		// row band: pass 1 stores the band's own elements (one contiguous slice); pass 2 fills the rest of its
 		// rows with conjugates of elements stored in other rows, lanes walking the band rows at a fixed
@@ -796,13 +852,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	T ar[RT], ai[RT];
 	const V2* __restrict__ blk = s_blk + slot * TE;
 	const V2* __restrict__ own = cur + (static_cast<size_t>(n) << LOGE);  // full layout, N > 64 only
-	const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;    // Hermitian layout, N > 64 only
 	// -i[H, rho_n] from LDS: the diagonal, then flip by flip (each coefficient prepared once for all elements)
-	T hr[RT], hi[RT];
 #pragma unroll
 	for(int r = 0; r < RT; ++r) {
-		hr[r] = T(0);
-		hi[r] = T(0);
 		if(!HX_SLOT_LIVE(r)) continue;
 		int q, i, j;
 		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
@@ -826,10 +878,8 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			V2 x1;
 			if(TPA == 1 || rf < kTile) {
 				x1 = *reinterpret_cast<const V2*>(bb + (lb ^ (static_cast<unsigned>(rf) * sizeof(V2))));
-			} else if constexpr(HERM) {  // row i^m lies outside the band: its element from the packed block
-				bool cj;
-				x1 = ownh[herm_index(i ^ m, j, LOGN, cj)];
-				if(cj) x1.y = -x1.y;
+			} else if constexpr(HERM) {  // row i^m lies outside the band: its part was added before the staging
+				x1 = V2{T(0), T(0)};
 			} else {
 				x1 = own[e ^ rf];
 			}
