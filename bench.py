@@ -395,6 +395,8 @@ def main(argv=None):
                     help="skip the FP32 / split-FP32 / batched lines measured beside the primary workload")
     ap.add_argument("--form-steps", type=int, default=500, help="timed steps of each single-problem form")
     ap.add_argument("--batch-steps", type=int, default=30, help="timed steps of each batched form")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the attainable-peak measurement (HBM read/copy, FP64/FP32 MFMA) reported in roofline")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args(argv)
@@ -489,6 +491,18 @@ def main(argv=None):
             "flops_per_launch": flops_launch, "mean_launch_ms": mean_launch_ms,
             "stage_ms": [float(x) for x in stage_ms],
             "hbm_GBs_stored_bytes": achieved}
+    if not args.no_peaks:
+        # what this GPU sustains, measured after the timed region by the library's own kernels: the fractions above
+        # are against datasheet peaks; these put the same achieved figure against a copy kernel (HBM) or a dense
+        # MFMA loop of the instruction the dense kernels issue
+        pk = hx.measure_peaks(0, 1 << 30, 5)
+        rf = line["roofline"]
+        rf["attainable"] = pk
+        if rf["bound"] == "hbm":
+            rf["frac_of_measured_copy"] = rf["achieved"] / pk["hbm_copy_gbs"]
+        else:
+            rf["frac_of_measured_mfma"] = rf["achieved"] / pk["mfma_f64_tflops" if args.precision == "fp64"
+                                                               else "mfma_f32_tflops"]
     traffic = None if args.dense_hamiltonian else _profiled_traffic(args.workload, args.precision, layout)
     if traffic is not None:
         line["roofline"]["traffic"] = traffic

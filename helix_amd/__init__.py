@@ -6,6 +6,6 @@ Native parts (built in-tree by ``python -m helix_amd.build``):
 
 This Python package is a thin ctypes front (``helix_amd.capi``) over the C entry points.
 """
-from .capi import FP32, FP64, SPLIT_FP32, Batch, HelixError, Model, Plan  # noqa: F401
+from .capi import FP32, FP64, SPLIT_FP32, Batch, HelixError, Model, Plan, measure_peaks  # noqa: F401
 
-__all__ = ["Model", "Plan", "Batch", "HelixError", "FP64", "FP32", "SPLIT_FP32"]
+__all__ = ["Model", "Plan", "Batch", "HelixError", "FP64", "FP32", "SPLIT_FP32", "measure_peaks"]

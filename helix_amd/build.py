@@ -22,7 +22,7 @@ CSRC = os.path.join(PKG, "csrc")
 ARCH = "gfx950"
 REFERENCE_EXAMPLE = "/root/reference/examples/cpp/legacy_spin_glass.cpp"
 
-HIP_SOURCES = ["heom_plan.hip", "mlp.hip"]
+HIP_SOURCES = ["heom_plan.hip", "mlp.hip", "peaks.hip"]
 HOST_SOURCES = ["host/heom_setup.cpp", "host/setup_capi.cpp", "host/helix_api.cpp", "host/helix_io.cpp"]
 
 HIPCC_FLAGS = [

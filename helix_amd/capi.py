@@ -30,6 +30,7 @@ HIP_SYMBOLS = [
     "helix_hip_batch_destroy", "helix_hip_plan_set_closure", "helix_hip_plan_set_closure_rates",
     "helix_hip_plan_layout", "helix_hip_batch_layout", "helix_hip_step_noalloc", "helix_hip_plan_set_terminator",
     "helix_hip_plan_set_terminator_gain", "helix_hip_get_ados", "helix_hip_batch_get_ados", "helix_hip_batch_launches",
+    "helix_hip_measure_peaks",
 ]
 SETUP_SYMBOLS = [
     "helix_model_legacy", "helix_model_spin_glass", "helix_model_spin_glass_random", "helix_model_sparse",
@@ -58,6 +59,12 @@ class CheckpointInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("n_ado", C.c_int32), ("n_modes", C.c_int32), ("n_baths", C.c_int32),
                 ("order", C.c_int32), ("precision", C.c_int32), ("steps", C.c_uint64), ("time", C.c_double),
                 ("dt", C.c_double), ("hierarchy_hash", C.c_uint64), ("problem_hash", C.c_uint64)]
+
+
+class Peaks(C.Structure):
+    """helix_hip_peaks"""
+    _fields_ = [("hbm_read_gbs", C.c_double), ("hbm_copy_gbs", C.c_double), ("mfma_f64_tflops", C.c_double),
+                ("mfma_f32_tflops", C.c_double), ("bytes", C.c_size_t), ("compute_units", C.c_int)]
 
 
 class Terminator(C.Structure):
@@ -130,6 +137,7 @@ def hip():
         L.helix_hip_plan_layout.argtypes = [vp, _ip]
         L.helix_hip_batch_layout.argtypes = [vp, _ip]
         L.helix_hip_batch_launches.argtypes = [vp, _ip]
+        L.helix_hip_measure_peaks.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(Peaks)]
         _hip = L
     return _hip
 
@@ -520,6 +528,15 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def measure_peaks(device: int = 0, nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """Attainable peaks of the device (helix_hip_measure_peaks): HBM read and copy GB/s, FP64 and FP32 MFMA TFLOP/s."""
+    out = Peaks()
+    _check(hip().helix_hip_measure_peaks(device, nbytes, reps, C.byref(out)), "measure_peaks")
+    return {"hbm_read_gbs": out.hbm_read_gbs, "hbm_copy_gbs": out.hbm_copy_gbs,
+            "mfma_f64_tflops": out.mfma_f64_tflops, "mfma_f32_tflops": out.mfma_f32_tflops,
+            "bytes": int(out.bytes), "compute_units": int(out.compute_units)}
 
 
 def pade(K: int):

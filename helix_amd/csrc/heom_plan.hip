@@ -40,6 +40,12 @@ int hip_fail(hipError_t e, const char* what)
 }
 }  // namespace
 
+// for the library's other translation units (peaks.hip)
+void set_last_error(const std::string& msg)
+{
+	g_last_error = msg;
+}
+
 #define HX_HIP(call)                                         \
 	do {                                                     \
 		hipError_t hx_e_ = (call);                           \
@@ -1191,9 +1197,10 @@ __device__ __forceinline__ void run_queue(const StageArgs* __restrict__ table, c
 	}
 }
 
+// two workgroups per CU: the one-ADO N = 64 complex64 body alone fits four (tile_waves), but with the other three
+// tile bodies in the same kernel the registers allow two
 template <typename T, bool SPLIT, bool LAST, int THREADS, bool HERM>
-__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(tile_waves<T, 6, THREADS, true, HERM>(),
-	tile_waves<T, 6, THREADS, true, HERM>()))) void
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(THREADS / 128, THREADS / 128))) void
 k_batch_persistent(const StageArgs* __restrict__ table, const BatchQueues bq, unsigned* __restrict__ counters,
 	unsigned* __restrict__ next_counters)
 {

@@ -255,6 +255,23 @@ int helix_hip_batch_layout(const helix_hip_batch* batch, int32_t* layout);
 int helix_hip_batch_launches(const helix_hip_batch* batch, int32_t* launches_per_step);
 void helix_hip_batch_destroy(helix_hip_batch* batch);
 
+/* Attainable peaks of one device, measured by the library's own kernels (no counterpart in the reference: the
+ * roofline fractions are quoted against datasheet peaks, and these put what the GPU sustains beside them).
+ * hbm_read_gbs: a streaming read of `bytes`; hbm_copy_gbs: a copy of `bytes`, counted as bytes read plus bytes
+ * written (both in one pass of short-lived workgroups, 8 KiB read or 4 KiB copied each); mfma_f64_tflops / mfma_f32_tflops: dense v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 issue (2048
+ * flops each) from 8 waves per CU, 8 independent accumulators per wave.  Each figure is the best of `reps` timings of 8
+ * back-to-back launches, after one untimed launch.  1 MiB <= bytes <= 1 TiB (rounded down to 16 B); allocates 2 x bytes of device memory
+ * for the call and synchronises the device. */
+typedef struct helix_hip_peaks {
+	double hbm_read_gbs;
+	double hbm_copy_gbs;
+	double mfma_f64_tflops;
+	double mfma_f32_tflops;
+	size_t bytes;        /* bytes each bandwidth kernel read */
+	int compute_units;   /* CUs the launches were sized for */
+} helix_hip_peaks;
+int helix_hip_measure_peaks(int device, size_t bytes, int reps, helix_hip_peaks* out);
+
 /* Message of the calling thread's last error ("" when none). */
 const char* helix_hip_last_error(void);
 
