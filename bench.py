@@ -59,16 +59,25 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(spec, steps: int = 1):
-    """One Taylor-4 step of the same workload on the FP64 C oracle (OpenMP over ADOs)."""
+def cpu_baseline(spec, target_s: float = 10.0, max_steps: int = 1000):
+    """Taylor-4 steps of the same workload on the FP64 C oracle (OpenMP over ADOs): one step, then, when that took
+    less than half of target_s, as many more as fill about target_s (at most max_steps in all), so that a small
+    workload's sample is not a few milliseconds."""
     from oracle import oracle as O
     spins, seed, L, dt, lam, gamma, T, K = spec
     cores = _cpu_cores()
     p = O.spin_glass_problem(spins, seed, lam=lam, gamma=gamma, T=T, K=K, L=L, threads=cores)
     r = p.initial()
     t0 = time.perf_counter()
-    p.taylor(r, dt, steps)
+    r = p.taylor(r, dt, 1)
     sec = time.perf_counter() - t0
+    steps = 1
+    if sec < 0.5 * target_s:
+        more = min(max_steps - 1, max(1, int(target_s / max(sec, 1e-6))))
+        t0 = time.perf_counter()
+        p.taylor(r, dt, more)
+        sec += time.perf_counter() - t0
+        steps += more
     return {"value": p.H * steps / sec, "unit": "ADO-steps/s", "cores": cores, "kind": "port",
             "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{steps} Taylor-4 step(s) of the full workload ({p.H} ADOs, N={p.N}) in {sec:.2f} s"}
