@@ -2209,9 +2209,6 @@ int validate(const helix_hip_problem* pr)
 	if(logN < 3 || logN > 12) return fail(HELIX_HIP_EINVAL, "n must be a power of two in [8, 4096]");
 	if(pr->n_ado <= 0 || pr->n_modes <= 0 || pr->n_baths <= 0 || pr->n_modes > 64 || pr->n_baths > 64)
 		return fail(HELIX_HIP_EINVAL, "hierarchy sizes out of range");
-	const unsigned long long elems = static_cast<unsigned long long>(pr->n_ado) << (2 * logN);
-	if(elems * static_cast<unsigned long long>(pr->n_baths) >= (1ull << 31))
-		return fail(HELIX_HIP_EINVAL, "state too large for 32-bit element indexing");
 	if(!pr->up || !pr->down || !pr->mode_bath || !pr->tier || !pr->down_comm || !pr->down_anti || !pr->self_coef ||
 		!pr->xi || !pr->xi_counter || !pr->couplings || !pr->hamiltonian.diag)
 		return fail(HELIX_HIP_EINVAL, "problem table pointer is null");
@@ -2250,6 +2247,13 @@ int validate(const helix_hip_problem* pr)
 		if(!pr->couplings[b].diag) return fail(HELIX_HIP_EINVAL, "coupling diag is null");
 		if(int rc = check_op(pr->couplings[b], "coupling")) return rc;
 	}
+	// element counters are 32-bit: over the state, and over the [B][H][N][N] buffer of the commutator pass that
+	// bit-flip and general sparse couplings use
+	bool coupling_pass = pr->couplings_csr != nullptr;
+	for(int b = 0; b < pr->n_baths; ++b) coupling_pass = coupling_pass || pr->couplings[b].nflips > 0;
+	const unsigned long long elems = static_cast<unsigned long long>(pr->n_ado) << (2 * logN);
+	if(elems * (coupling_pass ? static_cast<unsigned long long>(pr->n_baths) : 1ull) >= (1ull << 31))
+		return fail(HELIX_HIP_EINVAL, "state too large for 32-bit element indexing");
 	for(long q = 0; q < static_cast<long>(pr->n_ado) * pr->n_modes; ++q) {
 		if(pr->up[q] < 0 || pr->up[q] > pr->n_ado || pr->down[q] < 0 || pr->down[q] > pr->n_ado)
 			return fail(HELIX_HIP_EINVAL, "up/down index outside [0, n_ado]");

@@ -117,7 +117,9 @@ typedef struct helix_hip_problem {
 typedef struct helix_hip_plan helix_hip_plan;
 
 /* Uploads tables and operators, allocates the state (rho plus m-1 stage buffers) on `device`.
- * Replaces initialize() + the first-call initSparse() (src/liouville.cu:685-747). */
+ * Replaces initialize() + the first-call initSparse() (src/liouville.cu:685-747).
+ * Size limit: n_ado * N^2 < 2^31 elements (n_ado * N^2 * n_baths with bit-flip or sparse couplings, whose
+ * commutator pass keeps one buffer per bath); HELIX_HIP_EINVAL beyond. */
 int helix_hip_plan_create(const helix_hip_problem* problem, int device, helix_hip_plan** out);
 
 /* rho_0 <- rho0 (tier 0, [N][N][2]); every other ADO <- 0 (src/initialize.cu:287-307). */

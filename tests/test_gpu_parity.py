@@ -75,6 +75,23 @@ def test_rhs_parity_large_n_fp64(hx, oracle_mod, spins, L, kernel, monkeypatch):
     assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
 
 
+@pytest.mark.parametrize("spins", [11, 12])
+def test_rhs_parity_largest_n_fp64(hx, oracle_mod, spins):
+    """N = 2048 and 4096, the largest dimensions the C-ABI admits (tier 1: 23 and 25 ADOs, 1.5 and 6.7 GB of
+    state): the untiled stage kernel in the full layout, on a random non-Hermitian state."""
+    print(f"largest N: spins {spins}, N {1 << spins}", flush=True)
+    m = hx.Model.spin_glass(spins, seed=4, L=1, lam=0.2, gamma=1.1, T=0.9)
+    p = oracle_mod.spin_glass_problem(spins, 4, L=1, lam=0.2, gamma=1.1, T=0.9)
+    plan = hx.Plan(m)
+    assert plan.layout() == "full"  # the Hermitian layout needs the tiled kernel, which stops at N = 1024
+    rng = np.random.default_rng(spins)
+    s = rng.standard_normal((m.H, m.N, m.N)) + 1j * rng.standard_normal((m.H, m.N, m.N))
+    plan.set_state(s)
+    got = plan.rhs()
+    plan.close()
+    assert _rel(got, p.rhs(s)) <= FP64_TOL
+
+
 def test_precision_form_n256_full_size(hx):
     """Precision form at its full size (8 spins, L=3: 969 ADOs of 256x256, dt 0.005, 1000 steps):
     complex64 and split-FP32 against the plain FP64 GPU form, whose RHS is pinned to the oracle above."""
