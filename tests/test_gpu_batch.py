@@ -51,6 +51,24 @@ def test_batch_against_oracle(hx, oracle_mod):
         assert float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))) <= FP64_TOL
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_ragged_with_depth_zero_and_one(hx, oracle_mod, persistent):
+    """A ragged batch: a one-ADO instance (tiers 0..0), a tier-1 one, and deeper ones of every block size
+    (N = 8, 16, 32, 64), per-size and persistent launches; each instance against the oracle over 50 steps."""
+    cases = [(3, 0, 0), (6, 1, 0), (4, 2, 1), (5, 3, 3), (6, 4, 2), (3, 5, 4)]
+    models = [hx.Model.spin_glass(sp, sd, L=L, dt=0.01) for sp, sd, L in cases]
+    batch = hx.Batch(models, graph=True, autotune=False, persistent=persistent)
+    batch.set_initial()
+    batch.step(50)
+    for q, (sp, sd, L) in enumerate(cases):
+        o = oracle_mod.spin_glass_problem(sp, sd, L=L)
+        ref = o.taylor(o.initial(), 0.01, 50)
+        got = batch.state(q)
+        assert got.shape == ref.shape
+        assert float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))) <= FP64_TOL, f"instance {q}: {sp} spins, L={L}"
+    batch.close()
+
+
 def test_batch_graph_replays_match_eager(hx):
     seeds = list(range(1020, 1028))
     g = hx.Batch(_models(hx, seeds), graph=True, autotune=False)

@@ -75,6 +75,27 @@ def test_rhs_parity_large_n_fp64(hx, oracle_mod, spins, L, kernel, monkeypatch):
     assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
 
 
+@pytest.mark.parametrize("kernel", ["tiled", "generic"])
+@pytest.mark.parametrize("spins", [3, 6])
+def test_depth_zero_hierarchy(hx, oracle_mod, spins, kernel, monkeypatch):
+    """Tiers 0..0: one ADO and no neighbours (the smallest hierarchy; only -i[H, rho] and the Xi terms act), RHS and
+    200 steps against the oracle."""
+    if kernel == "generic":
+        monkeypatch.setenv("HELIX_HIP_GENERIC_STAGE", "1")
+    m = hx.Model.spin_glass(spins, seed=6, L=0, dt=0.01)
+    assert m.H == 1
+    p = oracle_mod.spin_glass_problem(spins, 6, L=0)
+    plan = hx.Plan(m)
+    s = _rand_state(1, m.N, 41)
+    plan.set_state(s)
+    assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
+    plan.set_initial()
+    plan.step(200)
+    ref = p.taylor(p.initial(), 0.01, 200)
+    assert _rel(plan.get_state(), ref) <= FP64_TOL
+    plan.close()
+
+
 @pytest.mark.parametrize("spins", [11, 12])
 def test_rhs_parity_largest_n_fp64(hx, oracle_mod, spins):
     """N = 2048 and 4096, the largest dimensions the C-ABI admits (tier 1: 23 and 25 ADOs, 1.5 and 6.7 GB of
