@@ -1258,8 +1258,18 @@ struct Mfma<float> {
 
 constexpr int kDenseThreads = 256;
 
+// Whole-block dense stage: at N = 64 its two padded matrices fill the LDS, so a CU holds one workgroup, and 1024
+// threads (16 waves, one 16 x 16 output tile each, four elements each in the neighbour terms) keep 4 waves on every
+// SIMD to hide the MFMA chains and the neighbour round trips (1.47x the 256-thread shape in FP64, 1.50x complex64).
+// Smaller blocks keep 256 threads (N = 32 at 1024 threads: one element each, 12 waves without a tile, 1.2x slower).
+template <int LOGN>
+constexpr int dense_threads()
+{
+	return LOGN >= 6 ? 1024 : kDenseThreads;
+}
+
 template <typename T, bool SPLIT, bool LAST, int LOGN, bool SZ>
-__global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a)
+__global__ __launch_bounds__(dense_threads<LOGN>()) void k_stage_dense(const StageArgs a)
 {
 	using V2 = typename C2t<T>::type;
 	using MF = Mfma<T>;
@@ -1268,10 +1278,12 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	constexpr int LOGE = 2 * LOGN;
 	constexpr int E = 1 << LOGE;
 	constexpr int P = N + 1;
-	constexpr int EPT = E / kDenseThreads;
+	constexpr int DT = dense_threads<LOGN>();
+	constexpr int NW = DT / 64;  // waves
+	constexpr int EPT = E / DT;
 	constexpr int NT = N / 16;
 	constexpr int TILES = NT * NT;
-	constexpr int MAXT = (TILES + 3) / 4;
+	constexpr int MAXT = (TILES + NW - 1) / NW;
 	__shared__ V2 s_rho[N * P];
 	__shared__ V2 s_h[N * P];
 	__shared__ V2 s_hlo[SPLIT ? N * P : 1];
@@ -1290,15 +1302,15 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	const V2* __restrict__ hl = static_cast<const V2*>(a.hdense_lo);
 
 	if constexpr(SZ) {
-		for(int q = tid; q < N; q += kDenseThreads) s_selfx[q] = a.selfx[q];
+		for(int q = tid; q < N; q += DT) s_selfx[q] = a.selfx[q];
 	} else {
-		for(int q = tid; q < B * N; q += kDenseThreads) s_v[q] = a.vdiag[q];
+		for(int q = tid; q < B * N; q += DT) s_v[q] = a.vdiag[q];
 	}
-	for(int q = tid; q < 2 * M; q += kDenseThreads) {
+	for(int q = tid; q < 2 * M; q += DT) {
 		const int v = (q < M) ? a.up[n * M + q] : a.down[n * M + (q - M)];
 		s_nb[q] = (v < H) ? v : -1;
 	}
-	for(int e = tid; e < E; e += kDenseThreads) {
+	for(int e = tid; e < E; e += DT) {
 		const int i = e >> LOGN, j = e & (N - 1);
 		s_rho[i * P + j] = own[e];
 		s_h[i * P + j] = hd[e];
@@ -1309,36 +1321,55 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	// G = H rho - rho H on MFMA
 	const int wave = tid >> 6, lane = tid & 63;
 	const int li = lane & 15, lk = lane >> 4;
+	// at N = 64 each tile accumulates alternate K steps in two register pairs (four independent MFMA chains per
+	// wave), summed after; smaller blocks, whose stages are not bound by the chains, keep one pair
+	constexpr int KS = LOGN >= 6 ? 2 : 1;
 	acc4 gr[MAXT], gi[MAXT];
 #pragma unroll
 	for(int tt = 0; tt < MAXT; ++tt) {
-		const int t = wave + 4 * tt;
+		const int t = wave + NW * tt;
 		gr[tt] = acc4{};
 		gi[tt] = acc4{};
 		if(t >= TILES) continue;
 		const int I0 = (t / NT) * 16, J0 = (t % NT) * 16;
-		for(int k0 = 0; k0 < N; k0 += 4) {
-			const int kk = k0 + lk;
-			const V2 rb = s_rho[kk * P + J0 + li];
-			const V2 ra = s_rho[(I0 + li) * P + kk];
+		acc4 cr[KS], ci[KS];
 #pragma unroll
-			for(int part = 0; part < (SPLIT ? 2 : 1); ++part) {
-				const V2* __restrict__ hs = part ? s_hlo : s_h;
-				const V2 ha = hs[(I0 + li) * P + kk];
-				const V2 hb = hs[kk * P + J0 + li];
-				// H rho
-				gr[tt] = MF::mma(ha.x, rb.x, gr[tt]);
-				gi[tt] = MF::mma(ha.x, rb.y, gi[tt]);
-				// - rho H
-				gr[tt] = MF::mma(-ra.x, hb.x, gr[tt]);
-				gi[tt] = MF::mma(-ra.y, hb.x, gi[tt]);
-				if(!a.hreal) {
-					gr[tt] = MF::mma(-ha.y, rb.y, gr[tt]);
-					gi[tt] = MF::mma(ha.y, rb.x, gi[tt]);
-					gr[tt] = MF::mma(ra.y, hb.y, gr[tt]);
-					gi[tt] = MF::mma(-ra.x, hb.y, gi[tt]);
+		for(int h = 0; h < KS; ++h) {
+			cr[h] = acc4{};
+			ci[h] = acc4{};
+		}
+		for(int k0 = 0; k0 < N; k0 += 4 * KS) {
+#pragma unroll
+			for(int h = 0; h < KS; ++h) {
+				const int kk = k0 + 4 * h + lk;
+				const V2 rb = s_rho[kk * P + J0 + li];
+				const V2 ra = s_rho[(I0 + li) * P + kk];
+#pragma unroll
+				for(int part = 0; part < (SPLIT ? 2 : 1); ++part) {
+					const V2* __restrict__ hs = part ? s_hlo : s_h;
+					const V2 ha = hs[(I0 + li) * P + kk];
+					const V2 hb = hs[kk * P + J0 + li];
+					// H rho
+					cr[h] = MF::mma(ha.x, rb.x, cr[h]);
+					ci[h] = MF::mma(ha.x, rb.y, ci[h]);
+					// - rho H
+					cr[h] = MF::mma(-ra.x, hb.x, cr[h]);
+					ci[h] = MF::mma(-ra.y, hb.x, ci[h]);
+					if(!a.hreal) {
+						cr[h] = MF::mma(-ha.y, rb.y, cr[h]);
+						ci[h] = MF::mma(ha.y, rb.x, ci[h]);
+						cr[h] = MF::mma(ra.y, hb.y, cr[h]);
+						ci[h] = MF::mma(-ra.x, hb.y, ci[h]);
+					}
 				}
 			}
+		}
+		gr[tt] = cr[0];
+		gi[tt] = ci[0];
+#pragma unroll
+		for(int h = 1; h < KS; ++h) {
+			gr[tt] += cr[h];
+			gi[tt] += ci[h];
 		}
 	}
 	__syncthreads();  // every MFMA operand read of rho is done
@@ -1347,7 +1378,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	const double2 sf = a.selfc[n];
 #pragma unroll
 	for(int tt = 0; tt < MAXT; ++tt) {
-		const int t = wave + 4 * tt;
+		const int t = wave + NW * tt;
 		if(t >= TILES) continue;
 		const int I0 = (t / NT) * 16, J0 = (t % NT) * 16;
 #pragma unroll
@@ -1385,7 +1416,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	T ar[EPT], ai[EPT];
 #pragma unroll
 	for(int r = 0; r < EPT; ++r) {
-		const int e = tid + kDenseThreads * r;
+		const int e = tid + DT * r;
 		const V2 c = s_rho[(e >> LOGN) * P + (e & (N - 1))];
 		ar[r] = c.x;
 		ai[r] = c.y;
@@ -1400,7 +1431,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 		const V2* __restrict__ src = cur + (static_cast<size_t>(idx) << LOGE);
 		V2 x[EPT];
 #pragma unroll
-		for(int r = 0; r < EPT; ++r) x[r] = src[tid + kDenseThreads * r];
+		for(int r = 0; r < EPT; ++r) x[r] = src[tid + DT * r];
 		if constexpr(SZ) {
 			const int mask = a.sz_mask[b];
 			const double two_alpha = 2.0 * a.sz_alpha[b];
@@ -1411,7 +1442,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 			}
 #pragma unroll
 			for(int r = 0; r < EPT; ++r) {
-				const int e = tid + kDenseThreads * r;
+				const int e = tid + DT * r;
 				const int bi = (e >> LOGN) & mask, bj = e & mask;
 				const bool differ = bi != bj;
 				const double sg = bi ? two_alpha : -two_alpha;
@@ -1429,7 +1460,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 			const double* __restrict__ vb = s_v + b * N;
 #pragma unroll
 			for(int r = 0; r < EPT; ++r) {
-				const int e = tid + kDenseThreads * r;
+				const int e = tid + DT * r;
 				const double vi = vb[e >> LOGN], vj = vb[e & (N - 1)];
 				const double d = vi - vj, s = vi + vj;
 				cmadd<T, SPLIT>(ar[r], ai[r], wc.x * d + wa.x * s, wc.y * d + wa.y * s, x[r].x, x[r].y);
@@ -1441,7 +1472,7 @@ __global__ __launch_bounds__(kDenseThreads) void k_stage_dense(const StageArgs a
 	V2* __restrict__ out = static_cast<V2*>(a.out);
 #pragma unroll
 	for(int r = 0; r < EPT; ++r) {
-		const int e = tid + kDenseThreads * r;
+		const int e = tid + DT * r;
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
 		const size_t g = (static_cast<size_t>(n) << LOGE) + e;
 		if constexpr(!LAST) {
@@ -2330,9 +2361,9 @@ void launch_dense_v(helix_hip_plan* p, const StageArgs& a, bool last)
 	if constexpr(LOGN <= 6) {
 		const unsigned grid = static_cast<unsigned>(((p->H + kXcds - 1) / kXcds) * kXcds);
 		if(last)
-			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, true, LOGN, SZ>), dim3(grid), dim3(dense_threads<LOGN>()), 0, p->stream, a);
 		else
-			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(kDenseThreads), 0, p->stream, a);
+			hipLaunchKernelGGL((k_stage_dense<T, SPLIT, false, LOGN, SZ>), dim3(grid), dim3(dense_threads<LOGN>()), 0, p->stream, a);
 	} else {
 		constexpr int NTT = (1 << LOGN) / kDT;
 		const int ngroups = p->H * NTT * NTT;
