@@ -1518,6 +1518,32 @@ __global__ __launch_bounds__(dense_threads<LOGN>()) void k_stage_dense(const Sta
 
 	const T sc = static_cast<T>(a.scale);
 	V2* __restrict__ out = static_cast<V2*>(a.out);
+	if constexpr(LAST) {
+		if(!(a.rho0_f64 != nullptr && n == 0)) {
+			// rho_new = rho + t_1 + ... + t_m with the loads of all the thread's elements issued together, first rho,
+			// then each earlier Taylor term, and the stores last (per element, one load and one wait per term made
+			// the last stage 1.35x an inner one)
+			V2 rv[EPT];
+#pragma unroll
+			for(int r = 0; r < EPT; ++r)
+				rv[r] = ld_stream<T>(static_cast<const V2*>(a.rho) + (static_cast<size_t>(n) << LOGE) + tid + DT * r);
+			for(int q = 0; q < a.ntprev; ++q) {
+				const V2* __restrict__ tp = static_cast<const V2*>(a.tprev[q]) + (static_cast<size_t>(n) << LOGE);
+				V2 t[EPT];
+#pragma unroll
+				for(int r = 0; r < EPT; ++r) t[r] = ld_stream<T>(tp + tid + DT * r);
+#pragma unroll
+				for(int r = 0; r < EPT; ++r) {
+					rv[r].x += t[r].x;
+					rv[r].y += t[r].y;
+				}
+			}
+#pragma unroll
+			for(int r = 0; r < EPT; ++r)
+				st_stream<T>(out + (static_cast<size_t>(n) << LOGE) + tid + DT * r, rv[r].x + ar[r] * sc, rv[r].y + ai[r] * sc);
+			return;
+		}
+	}
 #pragma unroll
 	for(int r = 0; r < EPT; ++r) {
 		const int e = tid + DT * r;
