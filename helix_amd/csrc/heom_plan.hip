@@ -83,6 +83,24 @@ __device__ __forceinline__ typename C2t<T>::type ld_stream(const typename C2t<T>
 	return typename C2t<T>::type{t.x, t.y};
 }
 
+// two consecutive complex64 elements (16-byte aligned), streamed as above
+__device__ __forceinline__ float4 ld_stream4(const float2* p)
+{
+	typedef float ev4 __attribute__((ext_vector_type(4)));
+	const ev4 t = __builtin_nontemporal_load(reinterpret_cast<const ev4*>(p));
+	return float4{t.x, t.y, t.z, t.w};
+}
+__device__ __forceinline__ void st_stream4(float2* p, const float4& v)
+{
+	typedef float ev4 __attribute__((ext_vector_type(4)));
+	ev4 t;
+	t.x = v.x;
+	t.y = v.y;
+	t.z = v.z;
+	t.w = v.w;
+	__builtin_nontemporal_store(t, reinterpret_cast<ev4*>(p));
+}
+
 // Hermitian layout (include/helix_hip.h): stored index of full element (i, j) of an N x N block, and
 // whether the stored value must be conjugated.
 // N <= 64 (whole blocks in a workgroup): ordered by x = i XOR j.  Main part [N][N/2], element (x, k):
@@ -540,13 +558,13 @@ __device__ __forceinline__ void pmadd(float& ar, float& ai, const float4& w, flo
 // whole blocks: slots r < EPT are main elements q = lt + TA r = x N/2 + k (row x = i XOR j, see herm_index);
 // slot EPT is the tail element q = N^2/2 + lt (the diagonal element i = j = N/2 + lt), held by threads
 // lt < N/2.  Row bands: the band's slice of the [N][N/2] main part and its rows' tail elements.
-template <int LOGN, int TA, int EPT, bool HERM>
+template <int LOGN, int TA, int EPT, bool HERM, bool PAIR = false>
 __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int& i, int& j)
 {
 	constexpr int N = 1 << LOGN;
 	if constexpr(HERM && LOGN <= 6) {
 		if(r < EPT) {
-			q = lt + TA * r;
+			q = PAIR ? 2 * (lt + TA * (r >> 1)) + (r & 1) : lt + TA * r;
 			const int x = q >> (LOGN - 1), k = q & (N / 2 - 1);
 			i = (x == 0) ? k : herm_insert0(k, herm_high_bit(x));
 			j = i ^ x;
@@ -596,6 +614,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	constexpr bool SPREAD_LDS = LOGN <= 8;  // N >= 512: computed, the LDS of two complex128 workgroups is full
 	using PW = PatternWeight<T, SPLIT>;
 	using WT = typename PW::type;
+	// complex64 N = 64 sigma^z blocks: a thread's main slots 2p, 2p + 1 are consecutive stored elements of one
+	// row x (rows hold N/2 elements), so each global load and store of a pair moves 16 bytes per lane
+	constexpr bool PAIR = sizeof(T) == 4 && HERM && TPA == 1 && LOGN == 6 && TABLE && EPT % 2 == 0;
 	static_assert(!HERM || TA >= (TPA == 1 ? N / 2 : R), "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
@@ -622,11 +643,21 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	V2 ownv[RT];
 	if constexpr(HERM && TPA == 1) {
 #pragma unroll
+		for(int r = 0; r < RT; ++r) ownv[r] = V2{T(0), T(0)};
+#pragma unroll
 		for(int r = 0; r < RT; ++r) {
-			ownv[r] = V2{T(0), T(0)};
 			if(!live || !HX_SLOT_LIVE(r)) continue;
 			int q, i, j;
-			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+			tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
+			if constexpr(PAIR) {
+				if(r < EPT) {
+					if(r & 1) continue;  // loaded with slot r - 1
+					const float4 v = *reinterpret_cast<const float4*>(cur + static_cast<size_t>(n) * PE + q);
+					ownv[r] = V2{v.x, v.y};
+					ownv[r + 1] = V2{v.z, v.w};
+					continue;
+				}
+			}
 			ownv[r] = cur[static_cast<size_t>(n) * PE + q];
 		}
 	}
@@ -720,7 +751,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		qb[r] = 0;
 		if(!HX_SLOT_LIVE(r)) continue;
 		int qe, i, j;
-		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+		tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 		qb[r] = static_cast<unsigned>(qe) * sizeof(V2);
 	}
 	// Whole Hermitian blocks (rows of one x = i XOR j): a neighbour without an anticommutator weight has a
@@ -746,6 +777,15 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				if(!HX_SLOT_LIVE(r)) continue;
 				if constexpr(SKIP)
 					if(!anti && (r == EPT || !((qb[r] >> (QSH + (shb[u] >> 1))) & 1u))) continue;
+				if constexpr(PAIR) {
+					if(r < EPT) {
+						if(r & 1) continue;  // loaded with slot r - 1 (same row x, same skip decision)
+						const float4 v = *reinterpret_cast<const float4*>(src + qb[r]);
+						xb[u][r] = V2{v.x, v.y};
+						xb[u][r + 1] = V2{v.z, v.w};
+						continue;
+					}
+				}
 				xb[u][r] = *reinterpret_cast<const V2*>(src + qb[r]);
 			}
 		}
@@ -798,7 +838,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				for(int r = 0; r < RT; ++r) {
 					if(!HX_SLOT_LIVE(r)) continue;
 					int q, i, j;
-					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+					tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 					const int pa = i ^ m, dp = (j - pa) & (N - 1);
 					const V2 x1 = (dp < N / 2) ? ownh[(pa << (LOGN - 1)) | dp] : s_blk[(r < EPT) ? lt + TA * r : R * (N / 2) + lt];
 					wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x, x1.y);
@@ -818,7 +858,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			for(int r = 0; r < RT; ++r) {
 				if(!HX_SLOT_LIVE(r)) continue;
 				int q, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 				s_blk[((i - r0) << LOGN) | j] = ownh[q];
 			}
 			for(int t = tid; t < kTile; t += THREADS) {
@@ -839,7 +879,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			for(int r = 0; r < RT; ++r) {
 				if(!HX_SLOT_LIVE(r)) continue;
 				int q, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 				const V2 v = ownv[r];
 				s_blk[slot * TE + ((i << LOGN) | j)] = v;
 				if(i != j) s_blk[slot * TE + ((j << LOGN) | i)] = V2{v.x, -v.y};
@@ -863,7 +903,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	for(int r = 0; r < RT; ++r) {
 		if(!HX_SLOT_LIVE(r)) continue;
 		int q, i, j;
-		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+		tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 		const V2 c = blk[((i << LOGN) | j) - ebase];
 		rmadd<T, SPLIT>(hr[r], hi[r], s_hd[i] - s_hd[j], c.x, c.y);
 	}
@@ -875,7 +915,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		for(int r = 0; r < RT; ++r) {
 			if(!HX_SLOT_LIVE(r)) continue;
 			int q, i, j;
-			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, q, i, j);
+			tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 			const int e = (i << LOGN) | j;
 			const int le = e - ebase;
 			// LDS gathers by byte offset: (le ^ f) * size == (le * size) ^ (f * size), one XOR per gather
@@ -909,7 +949,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			pw[r] = 0;
 			if(!HX_SLOT_LIVE(r)) continue;
 			int qe, i, j;
-			tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+			tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 			pw[r] = SPREAD_LDS ? (s_spread[i] << 1) | s_spread[j] : (spread_bits(i) << 1) | spread_bits(j);
 		}
 		for(int c0 = 0;; c0 += NB) {
@@ -952,7 +992,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				for(int r = 0; r < RT; ++r) {
 					if(!HX_SLOT_LIVE(r)) continue;
 					int qe, i, j;
-					tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+					tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 					x[u][r] = src[qe];
 				}
 			}
@@ -974,7 +1014,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					for(int r = 0; r < RT; ++r) {
 						if(!HX_SLOT_LIVE(r)) continue;
 						int qe, i, j;
-						tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+						tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 						const int bi = i & mask, bj = j & mask;
 						const bool differ = bi != bj;
 						const Weight<T, SPLIT> w = ws.select(differ, wd, bi == 0);
@@ -990,7 +1030,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					for(int r = 0; r < RT; ++r) {
 						if(!HX_SLOT_LIVE(r)) continue;
 						int qe, i, j;
-						tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+						tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 						const double vi = vb[i], vj = vb[j];
 						const double d = vi - vj, sm = vi + vj;
 						cmadd<T, SPLIT>(ar[r], ai[r], wc[u].x * d + wa[u].x * sm, wc[u].y * d + wa[u].y * sm, x[u][r].x,
@@ -1011,11 +1051,12 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
-#pragma unroll 2
+	constexpr int EPI_UNROLL = PAIR ? RT : 2;  // PAIR: whole unroll, the odd slot stores its pair
+#pragma unroll EPI_UNROLL
 	for(int r = 0; r < RT; ++r) {
 		if(!HX_SLOT_LIVE(r)) continue;
 		int qe, i, j;
-		tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+		tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 		const int e = (i << LOGN) | j;
 		double wr = sf.x, wi = sf.y;
 		if constexpr(SZ) {
@@ -1048,10 +1089,20 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			// complex64 inner stages: ordinary stores measured 3% faster than non-temporal ones on the N = 64
 			// Hermitian tile (their output is the next stage's input); complex128 keeps the streaming store (3.5% the
 			// other way)
-			if constexpr(sizeof(T) == 4)
+			if constexpr(PAIR) {
+				if(r < EPT) {
+					ar[r] = tr;  // the pair (r - 1, r) is stored by its odd slot
+					ai[r] = ti;
+					if(r & 1)
+						*reinterpret_cast<float4*>(out + g - 1) = float4{ar[r - 1], ai[r - 1], tr, ti};
+				} else {
+					out[g] = V2{tr, ti};
+				}
+			} else if constexpr(sizeof(T) == 4) {
 				out[g] = V2{tr, ti};
-			else
+			} else {
 				st_stream<T>(out + g, tr, ti);
+			}
 		} else {
 			const int nt = a.ntprev;
 			if(a.rho0_f64 != nullptr && n == 0) {
@@ -1074,19 +1125,56 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				ar[r] = tr;  // t_m of the slot, summed below
 				ai[r] = ti;
 			} else {
-				V2 rv = (nt == 0) ? c : ld_stream<T>(static_cast<const V2*>(a.rho) + g);
-				for(int q = 0; q + 1 < nt; ++q) {
-					const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
-					rv.x += t.x;
-					rv.y += t.y;
+				bool paired = false;
+				if constexpr(PAIR) {
+					if(r < EPT) {
+						paired = true;
+						// the odd slot sums and stores the pair (r - 1, r) with 16-byte accesses, each element's terms
+						// added in the order of the single-element sum below
+						ar[r] = tr;
+						ai[r] = ti;
+						if(r & 1) {
+							int q0, i0, j0;
+							tile_elem<LOGN, TA, EPT, HERM, PAIR>(r - 1, lt, ebase, q0, i0, j0);
+							const V2 c0 = blk[((i0 << LOGN) | j0) - ebase];
+							const size_t g0 = g - 1;
+							float4 rv = (nt == 0) ? float4{c0.x, c0.y, c.x, c.y} : ld_stream4(static_cast<const V2*>(a.rho) + g0);
+							for(int q = 0; q + 1 < nt; ++q) {
+								const float4 t = ld_stream4(static_cast<const V2*>(a.tprev[q]) + g0);
+								rv.x += t.x;
+								rv.y += t.y;
+								rv.z += t.z;
+								rv.w += t.w;
+							}
+							if(nt > 0) {
+								rv.x += c0.x;
+								rv.y += c0.y;
+								rv.z += c.x;
+								rv.w += c.y;
+							}
+							rv.x += ar[r - 1];
+							rv.y += ai[r - 1];
+							rv.z += tr;
+							rv.w += ti;
+							st_stream4(out + g0, rv);
+						}
+					}
 				}
-				if(nt > 0) {
-					rv.x += c.x;
-					rv.y += c.y;
+				if(!paired) {
+					V2 rv = (nt == 0) ? c : ld_stream<T>(static_cast<const V2*>(a.rho) + g);
+					for(int q = 0; q + 1 < nt; ++q) {
+						const V2 t = ld_stream<T>(static_cast<const V2*>(a.tprev[q]) + g);
+						rv.x += t.x;
+						rv.y += t.y;
+					}
+					if(nt > 0) {
+						rv.x += c.x;
+						rv.y += c.y;
+					}
+					rv.x += tr;
+					rv.y += ti;
+					st_stream<T>(out + g, rv.x, rv.y);
 				}
-				rv.x += tr;
-				rv.y += ti;
-				st_stream<T>(out + g, rv.x, rv.y);
 			}
 		}
 	}
@@ -1101,7 +1189,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				rv[r] = V2{T(0), T(0)};
 				if(!HX_SLOT_LIVE(r)) continue;
 				int qe, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 				g[r] = static_cast<unsigned>(n) * static_cast<unsigned>(PE) + static_cast<unsigned>(qe);
 				rv[r] = (nt == 0) ? blk[((i << LOGN) | j) - ebase] : ld_stream<T>(static_cast<const V2*>(a.rho) + g[r]);
 			}
@@ -1120,7 +1208,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			for(int r = 0; r < RT; ++r) {
 				if(!HX_SLOT_LIVE(r)) continue;
 				int qe, i, j;
-				tile_elem<LOGN, TA, EPT, HERM>(r, lt, ebase, qe, i, j);
+				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 				if(nt > 0) {
 					const V2 c = blk[((i << LOGN) | j) - ebase];
 					rv[r].x += c.x;
@@ -4019,6 +4107,7 @@ int helix_hip_batch_create(const helix_hip_problem* const* problems, int count, 
 		b->logN.push_back(lg);
 		b->H.push_back(pr->n_ado);
 		b->pe.push_back(pe);
+		b->total_elems += b->total_elems & 1;  // instance states start 16-byte aligned (paired complex64 accesses)
 		b->elem_off.push_back(b->total_elems);
 		b->total_elems += static_cast<size_t>(pr->n_ado) * pe;
 		b->total_ado += pr->n_ado;
