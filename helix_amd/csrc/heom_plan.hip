@@ -577,7 +577,7 @@ __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int&
 		// row band of R = 4096/N rows from r0 = ebase / N: its stored main part is one contiguous slice
 		const int r0 = ebase >> LOGN;
 		if(r < EPT) {
-			const int qq = lt + TA * r;  // 0 .. 2047
+			const int qq = PAIR ? 2 * (lt + TA * (r >> 1)) + (r & 1) : lt + TA * r;  // 0 .. 2047
 			q = (r0 << (LOGN - 1)) + qq;
 			i = r0 + (qq >> (LOGN - 1));
 			j = (i + (qq & (N / 2 - 1))) & (N - 1);
@@ -614,9 +614,10 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	constexpr bool SPREAD_LDS = LOGN <= 8;  // N >= 512: computed, the LDS of two complex128 workgroups is full
 	using PW = PatternWeight<T, SPLIT>;
 	using WT = typename PW::type;
-	// complex64 N = 64 sigma^z blocks: a thread's main slots 2p, 2p + 1 are consecutive stored elements of one
-	// row x (rows hold N/2 elements), so each global load and store of a pair moves 16 bytes per lane
-	constexpr bool PAIR = sizeof(T) == 4 && HERM && TPA == 1 && LOGN == 6 && TABLE && EPT % 2 == 0;
+	// complex64 sigma^z tiles of whole N = 64 blocks and of row bands (N >= 128): a thread's main slots 2p, 2p + 1
+	// are consecutive stored elements of one stored row (N/2 elements: one x = i XOR j, or one row i of a band), so
+	// each global load and store of a pair moves 16 bytes per lane
+	constexpr bool PAIR = sizeof(T) == 4 && HERM && (TPA > 1 || LOGN == 6) && TABLE && EPT % 2 == 0;
 	static_assert(!HERM || TA >= (TPA == 1 ? N / 2 : R), "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
@@ -840,7 +841,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					int q, i, j;
 					tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
 					const int pa = i ^ m, dp = (j - pa) & (N - 1);
-					const V2 x1 = (dp < N / 2) ? ownh[(pa << (LOGN - 1)) | dp] : s_blk[(r < EPT) ? lt + TA * r : R * (N / 2) + lt];
+					const V2 x1 = (dp < N / 2) ? ownh[(pa << (LOGN - 1)) | dp] : s_blk[(r < EPT) ? q - (r0 << (LOGN - 1)) : R * (N / 2) + lt];
 					wmadd<T, SPLIT>(hr[r], hi[r], w, x1.x, x1.y);
 				}
 			}
@@ -859,6 +860,15 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 				if(!HX_SLOT_LIVE(r)) continue;
 				int q, i, j;
 				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, q, i, j);
+				if constexpr(PAIR) {
+					if(r < EPT) {
+						if(r & 1) continue;  // staged with slot r - 1: same row i, column j + 1 (mod N)
+						const float4 v = *reinterpret_cast<const float4*>(ownh + q);
+						s_blk[((i - r0) << LOGN) | j] = V2{v.x, v.y};
+						s_blk[((i - r0) << LOGN) | ((j + 1) & (N - 1))] = V2{v.z, v.w};
+						continue;
+					}
+				}
 				s_blk[((i - r0) << LOGN) | j] = ownh[q];
 			}
 			for(int t = tid; t < kTile; t += THREADS) {
@@ -1183,21 +1193,48 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		if(!(a.rho0_f64 != nullptr && n == 0)) {
 			unsigned g[RT];  // element offsets (a state holds fewer than 2^31 elements)
 			V2 rv[RT];
+			// slots' loads of one term: PAIR slots (2p, 2p + 1) as one 16-byte load
+			auto load_term = [&](const V2* __restrict__ base, V2 (&dst)[RT]) {
+#pragma unroll
+				for(int r = 0; r < RT; ++r) dst[r] = V2{T(0), T(0)};  // before the loads: a pair fills two slots
+#pragma unroll
+				for(int r = 0; r < RT; ++r) {
+					if(!HX_SLOT_LIVE(r)) continue;
+					if constexpr(PAIR) {
+						if(r < EPT) {
+							if(r & 1) continue;  // loaded with slot r - 1
+							const float4 v = ld_stream4(base + g[r]);
+							dst[r] = V2{v.x, v.y};
+							dst[r + 1] = V2{v.z, v.w};
+							continue;
+						}
+					}
+					dst[r] = ld_stream<T>(base + g[r]);
+				}
+			};
 #pragma unroll
 			for(int r = 0; r < RT; ++r) {
 				g[r] = 0;
-				rv[r] = V2{T(0), T(0)};
 				if(!HX_SLOT_LIVE(r)) continue;
 				int qe, i, j;
 				tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
 				g[r] = static_cast<unsigned>(n) * static_cast<unsigned>(PE) + static_cast<unsigned>(qe);
-				rv[r] = (nt == 0) ? blk[((i << LOGN) | j) - ebase] : ld_stream<T>(static_cast<const V2*>(a.rho) + g[r]);
+			}
+			if(nt == 0) {
+#pragma unroll
+				for(int r = 0; r < RT; ++r) {
+					rv[r] = V2{T(0), T(0)};
+					if(!HX_SLOT_LIVE(r)) continue;
+					int qe, i, j;
+					tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
+					rv[r] = blk[((i << LOGN) | j) - ebase];
+				}
+			} else {
+				load_term(static_cast<const V2*>(a.rho), rv);
 			}
 			for(int q = 0; q + 1 < nt; ++q) {
-				const V2* __restrict__ tp = static_cast<const V2*>(a.tprev[q]);
 				V2 t[RT];
-#pragma unroll
-				for(int r = 0; r < RT; ++r) t[r] = HX_SLOT_LIVE(r) ? ld_stream<T>(tp + g[r]) : V2{T(0), T(0)};
+				load_term(static_cast<const V2*>(a.tprev[q]), t);
 #pragma unroll
 				for(int r = 0; r < RT; ++r) {
 					rv[r].x += t[r].x;
@@ -1213,6 +1250,14 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					const V2 c = blk[((i << LOGN) | j) - ebase];
 					rv[r].x += c.x;
 					rv[r].y += c.y;
+				}
+				if constexpr(PAIR) {
+					if(r < EPT) {
+						if(r & 1)  // the pair (r - 1, r), both complete
+							st_stream4(out + g[r - 1], float4{rv[r - 1].x + ar[r - 1], rv[r - 1].y + ai[r - 1],
+								rv[r].x + ar[r], rv[r].y + ai[r]});
+						continue;
+					}
 				}
 				st_stream<T>(out + g[r], rv[r].x + ar[r], rv[r].y + ai[r]);
 			}

@@ -75,6 +75,40 @@ def test_rhs_parity_large_n_fp64(hx, oracle_mod, spins, L, kernel, monkeypatch):
     assert _rel(plan.rhs(), p.rhs(s)) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["fp32", "split_fp32"])
+@pytest.mark.parametrize("spins,L", [(6, 2), (7, 2), (8, 1), (9, 1), (10, 1)])
+def test_rhs_parity_precision_forms(hx, oracle_mod, spins, L, precision):
+    """complex64 and split-FP32 right-hand sides against the FP64 oracle, within FP32 rounding (normwise 1e-5; a
+    misplaced element is an O(1) error): whole N = 64 blocks and Hermitian row bands (N = 128, 256), whose threads
+    hold pairs of consecutive stored elements, and the full-layout row tiles of N = 512, 1024."""
+    m = hx.Model.spin_glass(spins, seed=2, L=L, lam=0.3, gamma=0.7, T=1.3, precision=precision)
+    p = oracle_mod.spin_glass_problem(spins, 2, L=L, lam=0.3, gamma=0.7, T=1.3)
+    plan = hx.Plan(m)
+    assert plan.layout() == ("hermitian" if spins <= 8 else "full")
+    s = _rand_state(m.H, m.N, 21)
+    plan.set_state(s)
+    err = _rel(plan.rhs(), p.rhs(s))
+    print(f"{precision} N={m.N} {m.H} ADOs: normwise rel err {err:.2e}")
+    assert err <= 1e-5
+    plan.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split_fp32"])
+def test_precision_form_row_band_trajectory(hx, oracle_mod, precision):
+    """7 spins (N = 128, row bands), tiers 0..1, 20 Taylor-4 steps: the precision form's rho_S, every stage's
+    paired Taylor sum included, against the FP64 oracle's trajectory, max|d rho_S| <= 1e-6."""
+    m = hx.Model.spin_glass(7, 2, L=1, dt=0.01, lam=0.3, gamma=0.7, T=1.3, precision=precision)
+    p = oracle_mod.spin_glass_problem(7, 2, L=1, lam=0.3, gamma=0.7, T=1.3)
+    plan = hx.Plan(m)
+    plan.set_initial()
+    plan.step(20)
+    ref = p.taylor(p.initial(), 0.01, 20)[0]
+    err = float(np.max(np.abs(plan.tier0() - ref)))
+    print(f"{precision}: N=128 {m.H} ADOs 20 steps max|d rho_S| = {err:.2e}")
+    assert err <= FORM_TOL
+    plan.close()
+
+
 @pytest.mark.parametrize("kernel", ["tiled", "generic"])
 @pytest.mark.parametrize("spins", [3, 6])
 def test_depth_zero_hierarchy(hx, oracle_mod, spins, kernel, monkeypatch):
