@@ -636,6 +636,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	const int ebase = (group % TPA) * kTile;  // first element of this row tile
 	const int lt = tid - slot * TA;
 	const int n = (group / TPA) * A + slot;  // this thread's ADO (uniform over TA threads)
+	// its first stored element; with one ADO per workgroup it is wave-uniform and stays in scalar registers (as a
+	// per-lane 64-bit value the complex64 N = 64 tile spilled it, 25 MB of scratch writes per launch)
+	const size_t nbase = static_cast<size_t>(A == 1 ? __builtin_amdgcn_readfirstlane(n) : n) * PE;
 	const bool live = n < H;
 	// the tail slot holds an element for lt < N/2 (whole blocks), or for lt < R in row tiles of the upper half
 	const bool tail_ok = HERM && (TPA == 1 ? lt < N / 2 : (lt < R && ebase < E / 2));
@@ -653,13 +656,13 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 			if constexpr(PAIR) {
 				if(r < EPT) {
 					if(r & 1) continue;  // loaded with slot r - 1
-					const float4 v = *reinterpret_cast<const float4*>(cur + static_cast<size_t>(n) * PE + q);
+					const float4 v = *reinterpret_cast<const float4*>(cur + nbase + q);
 					ownv[r] = V2{v.x, v.y};
 					ownv[r + 1] = V2{v.z, v.w};
 					continue;
 				}
 			}
-			ownv[r] = cur[static_cast<size_t>(n) * PE + q];
+			ownv[r] = cur[nbase + q];
 		}
 	}
 
@@ -808,7 +811,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		// d = (j - i) mod N, sits at (i - r0) N/2 + d, a tail owner (d = N/2, i < N/2) at R N/2 + (i - r0).  Each
 		// owner then adds h_f x (partner); the column-flip part of these flips follows from the staged tile.
 		const int r0 = ebase >> LOGN;
-		const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;
+		const V2* __restrict__ ownh = cur + nbase;
 		for(int f = 0; f < a.hflips; ++f) {
 			const int m = a.hmask[f];
 			if((m << LOGN) < kTile) continue;  // flip within the band: from the staged tile
@@ -854,7 +857,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		// column so that the loads of one column are contiguous
 		if(live) {
 			const int r0 = ebase >> LOGN;
-			const V2* __restrict__ ownh = cur + static_cast<size_t>(n) * PE;
+			const V2* __restrict__ ownh = cur + nbase;
 #pragma unroll
 			for(int r = 0; r < RT; ++r) {
 				if(!HX_SLOT_LIVE(r)) continue;
@@ -1094,7 +1097,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 		const V2 c = blk[e - ebase];
 		cmadd<T, SPLIT>(ar[r], ai[r], wr, wi, c.x, c.y);
 		const T tr = ar[r] * sc, ti = ai[r] * sc;
-		const size_t g = static_cast<size_t>(n) * PE + qe;
+		const size_t g = nbase + qe;
 		if constexpr(!LAST) {
 			// complex64 inner stages: ordinary stores measured 3% faster than non-temporal ones on the N = 64
 			// Hermitian tile (their output is the next stage's input); complex128 keeps the streaming store (3.5% the
