@@ -1058,9 +1058,10 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// (tprev[ntprev-1] == cur), and for order 1 so is rho.
 	// BATCHED: the last stage's sum rho + t_1 + ... + t_m issues the loads of all slots together, first rho, then each
 	// earlier Taylor term, and stores last, so no load waits behind another slot's store or sum (per slot, one load
-	// and one wait per term).  Kept for the complex64 row bands (three slots; N = 256 last stage 5.5% faster); the
-	// complex128 N = 64 block ran 5% slower that way (its last stage is bound by bytes, and the sum spills).
-	constexpr bool BATCHED = LAST && HERM && RT == 3 && TPA > 1 && sizeof(T) == 4;
+	// and one wait per term).  Kept for the complex64 row bands (three slots; N = 256 last stage 5.5% faster) and the
+	// paired complex64 N = 64 blocks (last stage 0.125 -> 0.122 ms, split-FP32 0.133 -> 0.123 ms); the complex128
+	// N = 64 block ran 5% slower that way (its last stage is bound by bytes, and the sum spills).
+	constexpr bool BATCHED = LAST && HERM && sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1));
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
