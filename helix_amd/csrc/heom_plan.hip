@@ -614,10 +614,10 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	constexpr bool SPREAD_LDS = LOGN <= 8;  // N >= 512: computed, the LDS of two complex128 workgroups is full
 	using PW = PatternWeight<T, SPLIT>;
 	using WT = typename PW::type;
-	// complex64 sigma^z tiles of whole N = 64 blocks and of row bands (N >= 128): a thread's main slots 2p, 2p + 1
-	// are consecutive stored elements of one stored row (N/2 elements: one x = i XOR j, or one row i of a band), so
-	// each global load and store of a pair moves 16 bytes per lane
-	constexpr bool PAIR = sizeof(T) == 4 && HERM && (TPA > 1 || LOGN == 6) && TABLE && EPT % 2 == 0;
+	// complex64 Hermitian tiles (whole blocks of one or several ADOs, row bands of N >= 128): a thread's main slots
+	// 2p, 2p + 1 are consecutive stored elements of one stored row (N/2 elements: one x = i XOR j, or one row i of a
+	// band), so each global load and store of a pair moves 16 bytes per lane
+	constexpr bool PAIR = sizeof(T) == 4 && HERM && EPT % 2 == 0 && (A > 1 || TABLE);
 	static_assert(!HERM || TA >= (TPA == 1 ? N / 2 : R), "every tail element needs a thread");
 	V2* __restrict__ s_blk = lds.blk;
 	double* __restrict__ s_v = lds.v;
@@ -1006,6 +1006,15 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 					if(!HX_SLOT_LIVE(r)) continue;
 					int qe, i, j;
 					tile_elem<LOGN, TA, EPT, HERM, PAIR>(r, lt, ebase, qe, i, j);
+					if constexpr(PAIR) {
+						if(r < EPT) {
+							if(r & 1) continue;  // loaded with slot r - 1
+							const float4 v = *reinterpret_cast<const float4*>(src + qe);
+							x[u][r] = V2{v.x, v.y};
+							x[u][r + 1] = V2{v.z, v.w};
+							continue;
+						}
+					}
 					x[u][r] = src[qe];
 				}
 			}
@@ -1061,7 +1070,7 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// and one wait per term).  Kept for the complex64 row bands (three slots; N = 256 last stage 5.5% faster) and the
 	// paired complex64 N = 64 blocks (last stage 0.125 -> 0.122 ms, split-FP32 0.133 -> 0.123 ms); the complex128
 	// N = 64 block ran 5% slower that way (its last stage is bound by bytes, and the sum spills).
-	constexpr bool BATCHED = LAST && HERM && sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1));
+	constexpr bool BATCHED = LAST && HERM && sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1 && A == 1));
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
