@@ -593,7 +593,7 @@ __device__ __forceinline__ void tile_elem(int r, int lt, int ebase, int& q, int&
 	}
 }
 
-template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM>
+template <typename T, bool SPLIT, bool LAST, int LOGN, int THREADS, bool SZ, bool HERM, bool SINGLE = true>
 __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, const TileLds<T>& lds,
 	const int tid = static_cast<int>(threadIdx.x))
 {
@@ -1068,9 +1068,10 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// BATCHED: the last stage's sum rho + t_1 + ... + t_m issues the loads of all slots together, first rho, then each
 	// earlier Taylor term, and stores last, so no load waits behind another slot's store or sum (per slot, one load
 	// and one wait per term).  Kept for the complex64 row bands (three slots; N = 256 last stage 5.5% faster) and the
-	// paired complex64 N = 64 blocks (last stage 0.125 -> 0.122 ms, split-FP32 0.133 -> 0.123 ms); the complex128
-	// N = 64 block ran 5% slower that way (its last stage is bound by bytes, and the sum spills).
-	constexpr bool BATCHED = LAST && HERM && sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1 && A == 1));
+	// paired complex64 N = 64 blocks of single problems (last stage 0.125 -> 0.122 ms, split-FP32 0.133 -> 0.123 ms;
+	// SINGLE: in the batched kernels the same sum spilled 28 bytes and their N = 64 last stage ran 13% slower); the
+	// complex128 N = 64 block ran 5% slower that way (its last stage is bound by bytes, and the sum spills).
+	constexpr bool BATCHED = LAST && HERM && sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1 && A == 1 && SINGLE));
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
@@ -1351,7 +1352,7 @@ k_batch_stage(const StageArgs* __restrict__ table, const int2* __restrict__ task
 	const int t = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
 	if(t >= ntasks) return;
 	const int2 tk = tasks[t];
-	tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM>(table[tk.x], tk.y,
+	tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM, false>(table[tk.x], tk.y,
 		TileLds<T>{s_blk, s_v, s_selfx, s_nb, s_cnt, s_mb, s_ka, s_hd, s_wt, s_spread});
 }
 
@@ -1386,7 +1387,7 @@ __device__ __forceinline__ void run_queue(const StageArgs* __restrict__ table, c
 		const int2 tk = bq.tasks[begin + t];
 		int tid = static_cast<int>(threadIdx.x);
 		asm volatile("" : "+v"(tid));
-		tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM>(table[tk.x], tk.y, lds, tid);
+		tile_body<T, SPLIT, LAST, LOGN, THREADS, true, HERM, false>(table[tk.x], tk.y, lds, tid);
 		__syncthreads();  // the next tile stages into the LDS this one read (and thread 0 rewrites s_task)
 	}
 }
