@@ -1070,9 +1070,9 @@ __device__ __forceinline__ void tile_body(const StageArgs& a, const int group, c
 	// and one wait per term).  Kept for the complex64 row bands (three slots; N = 256 last stage 5.5% faster) and the
 	// paired complex64 N = 64 blocks of single problems (last stage 0.125 -> 0.122 ms, split-FP32 0.133 -> 0.123 ms;
 	// SINGLE: in the batched kernels the same sum spilled 28 bytes and their N = 64 last stage ran 13% slower) and the
-	// complex128 N = 64 blocks of single problems (0.234 -> 0.228 ms; it spilled, and ran 5% slower, until the
-	// epilogue re-read its own element from LDS).
-	constexpr bool BATCHED = LAST && HERM && ((sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1 && A == 1 && SINGLE))) || (sizeof(T) == 8 && TPA == 1 && A == 1 && SINGLE));
+	// complex128 N = 64 blocks (single problems 0.234 -> 0.228 ms, 512-instance batch +0.8%; it spilled, and ran 5%
+	// slower, until the epilogue re-read its own element from LDS).
+	constexpr bool BATCHED = LAST && HERM && ((sizeof(T) == 4 && ((RT == 3 && TPA > 1) || (PAIR && TPA == 1 && A == 1 && SINGLE))) || (sizeof(T) == 8 && TPA == 1 && A == 1));
 	const T sc = static_cast<T>(a.scale);
 	const double2 sf = a.selfc[n];
 	V2* __restrict__ out = static_cast<V2*>(a.out);
