@@ -3426,10 +3426,19 @@ void helix_hip_plan_destroy(helix_hip_plan* p)
 int helix_hip_set_initial(helix_hip_plan* p, const double* rho0)
 {
 	if(!p || !rho0) return fail(HELIX_HIP_EINVAL, "null argument");
-	const size_t n2 = static_cast<size_t>(p->N) * p->N;
-	std::vector<double> state(static_cast<size_t>(p->H) * n2 * 2, 0.0);
-	std::memcpy(state.data(), rho0, n2 * 16);
-	return host_to_state(p, state.data());
+	// every ADO but tier 0 is zero: the state is cleared on the device and only tier 0's block crosses the bus (a
+	// host-built state of all H blocks kept the GPU idle for a large fraction of a second)
+	std::vector<double> st;
+	if(int rc = to_stored(rho0, 1, p->logN, p->herm, {}, st)) return rc;
+	const bool fp64 = p->precision == HELIX_HIP_FP64;
+	const size_t esize = fp64 ? 16 : 8;
+	HX_HIP(hipStreamSynchronize(p->stream));
+	HX_HIP(hipMemset(p->rho, 0, p->elems * esize));
+	char* block0 = static_cast<char*>(p->rho) + static_cast<size_t>(p->perm.empty() ? 0 : p->perm[0]) * p->pe * esize;
+	if(int rc = upload_stored(block0, st.data(), p->pe, fp64)) return rc;
+	if(!fp64) HX_HIP(hipMemcpy(p->rho0_f64, st.data(), p->pe * 16, hipMemcpyHostToDevice));
+	HX_HIP(hipDeviceSynchronize());
+	return 0;
 }
 
 int helix_hip_set_state(helix_hip_plan* p, const double* state)
